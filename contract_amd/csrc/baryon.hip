@@ -1,0 +1,341 @@
+// baryon.hip -- baryon two-point contractions on gfx950.
+//
+// Every contraction type contracts three propagators with two colour epsilons
+// and three spin matrices:
+//
+//   C(x) = sum Gij[i,j] Gkl[k,l] Gmn[m,n] ee(P_i[i,X], P_j[j,Y], P_n[n,Z])
+//
+// where type I<X>_J<Y>_N<Z> names the source-spin index each propagator
+// carries (include/contract_enum.h:1-7; the SWAP_IJ / SWAP_KL forms of
+// include/baryon.cuh:41-52,95-105 reduce to this table).  Instead of
+// evaluating C for one Gmn at a time, a site group computes the open-spin
+// correlator R[n][m] = dC/dGmn[m,n] (16 numbers), from which any insertion or
+// projector follows as C = sum_mn W[m,n] R[n,m]:
+//
+//   Z = m (IK_JL_NM, IL_JK_NM):  X1[a,d] = sum_{il} g cof(P_i[i,X], P_j[j,Y])[a,d]
+//                                R[n][m] = sum_ad P_n[n,m](a,d) X1[a,d]
+//   Z in {k,l} (the other four):  W_in[a,d] = sum_l g cof(A[., a-slot], P_n[n, b-slot])
+//                                R[n][m] = sum_i sum_ad M[., m](a,d) W_in[a,d]
+//
+// with M the propagator that carries m and A the other one of (P_i, P_j).
+// That costs ~45 (type 1) or ~180 (type 2) complex multiply-adds per lane for
+// all sixteen insertions, against ~100-200 per lane per single insertion in
+// the reference's lane loop (include/baryon.cuh:55-82).
+//
+// Lane q of a 16-lane site group holds R[n][m] for q = m*4 + n.
+
+#include "baryon.hpp"
+#include "device.hpp"
+#include "runtime.hpp"
+#include "spin.hpp"
+
+namespace contract_amd
+{
+
+enum : int { LK = 0, LL = 1, LM = 2 };
+__host__ __device__ constexpr int type_slot(int type, int prop)
+{
+  // (P_i, P_j, P_n) source letters for the six types, in enum order
+  constexpr int tab[6][3] = {{LK, LL, LM}, {LK, LM, LL}, {LL, LK, LM}, {LL, LM, LK}, {LM, LK, LL}, {LM, LL, LK}};
+  return tab[type][prop];
+}
+
+template <int T, typename C>
+__device__ __forceinline__ C baryon_open(const C *__restrict__ si, const C *__restrict__ sj, const C *__restrict__ sn,
+                                         int q, const SparseSpin &gij, const SparseSpin &gklT)
+{
+  constexpr int X = type_slot(T, 0), Y = type_slot(T, 1), Z = type_slot(T, 2);
+  C a[9], b[9];
+  if constexpr (Z == LM) {
+    const int i = q >> 2, l = q & 3;
+    const int j = sparse_col(gij, i), k = sparse_col(gklT, l);
+    const C w = cmul(sparse_val<C>(gij, i), sparse_val<C>(gklT, l));
+    load_block(a, si, i, X == LK ? k : l);
+    load_block(b, sj, j, Y == LK ? k : l);
+    C x1[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) x1[e] = czero<C>();
+    cof_acc(x1, a, b, w);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) x1[e] = group16_allreduce(x1[e]);
+    const int m = q >> 2, n = q & 3;
+    load_block(a, sn, n, m);
+    return dot9(a, x1);
+  } else {
+    constexpr bool carrier_j = (Y == LM);     // which of P_i / P_j carries m
+    constexpr int LA = carrier_j ? X : Y;     // source letter of the other one
+    const int i = q >> 2, n = q & 3;
+    const int j = sparse_col(gij, i);
+    const C gi = sparse_val<C>(gij, i);
+    C acc[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) acc[e] = czero<C>();
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int k = sparse_col(gklT, l);
+      const C w = cmul(gi, sparse_val<C>(gklT, l));
+      if constexpr (carrier_j)
+        load_block(a, si, i, LA == LK ? k : l);
+      else
+        load_block(a, sj, j, LA == LK ? k : l);
+      load_block(b, sn, n, Z == LK ? k : l);
+      cof_acc(acc, a, b, w);
+    }
+    C r[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if constexpr (carrier_j)
+        load_block(a, sj, j, m);
+      else
+        load_block(a, si, i, m);
+      r[m] = group_allreduce_bits<C, 12>(dot9(a, acc));
+    }
+    const int msel = q >> 2;
+    C out = r[0];
+    out = (msel == 1) ? r[1] : out;
+    out = (msel == 2) ? r[2] : out;
+    out = (msel == 3) ? r[3] : out;
+    return out;
+  }
+}
+
+template <typename C>
+__device__ __forceinline__ C baryon_open_rt(int type, const C *si, const C *sj, const C *sn, int q,
+                                            const SparseSpin &gij, const SparseSpin &gklT)
+{
+  switch (type) {
+  case 0: return baryon_open<0, C>(si, sj, sn, q, gij, gklT);
+  case 1: return baryon_open<1, C>(si, sj, sn, q, gij, gklT);
+  case 2: return baryon_open<2, C>(si, sj, sn, q, gij, gklT);
+  case 3: return baryon_open<3, C>(si, sj, sn, q, gij, gklT);
+  case 4: return baryon_open<4, C>(si, sj, sn, q, gij, gklT);
+  default: return baryon_open<5, C>(si, sj, sn, q, gij, gklT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in per-site kernel: correl[x] = sum_q W[q] R_q(x)   (q = m*4 + n)
+// baryon_two_point passes W = G(gamma_mn), baryon_general_two_point passes
+// W = project_mn (include/baryon_general.cuh:62-67).
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(WAVE) void baryon_site_kernel(BaryonSiteArgs a)
+{
+  using C = double2;
+  __shared__ C lds[3][SITES_PER_WAVE * SITE_LDS];
+  const int lane = threadIdx.x;
+  const int g = lane >> 4, q = lane & 15;
+  const size_t first = (size_t)blockIdx.x * SITES_PER_WAVE;
+  const size_t left = a.volume - first;
+  const int nsites = left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
+
+  stage_sites(lds[0], (const C *)a.propag_i, first, nsites, lane);
+  stage_sites(lds[1], (const C *)a.propag_j, first, nsites, lane);
+  stage_sites(lds[2], (const C *)a.propag_n, first, nsites, lane);
+  __syncthreads();
+
+  const C R = baryon_open<T, C>(lds[0] + g * SITE_LDS, lds[1] + g * SITE_LDS, lds[2] + g * SITE_LDS, q, a.gij,
+                                a.gklT);
+  const C w = cmake<C>(a.w_re[q], a.w_im[q]);
+  const C c = group16_allreduce(cmul(w, R));
+  if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
+}
+
+void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream)
+{
+  const size_t blocks = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
+  if (blocks == 0) return;
+  const dim3 grid((unsigned)blocks), block(WAVE);
+  switch (type) {
+  case 0: hipLaunchKernelGGL(baryon_site_kernel<0>, grid, block, 0, stream, a); break;
+  case 1: hipLaunchKernelGGL(baryon_site_kernel<1>, grid, block, 0, stream, a); break;
+  case 2: hipLaunchKernelGGL(baryon_site_kernel<2>, grid, block, 0, stream, a); break;
+  case 3: hipLaunchKernelGGL(baryon_site_kernel<3>, grid, block, 0, stream, a); break;
+  case 4: hipLaunchKernelGGL(baryon_site_kernel<4>, grid, block, 0, stream, a); break;
+  default: hipLaunchKernelGGL(baryon_site_kernel<5>, grid, block, 0, stream, a); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused kernel: open-spin correlator summed over contraction terms, projected
+// onto momenta per timeslice with FP64 MFMA.
+//
+// Work item = a contiguous run of sites inside one timeslice (and parity for
+// even-odd order).  Per 4-site step the wave forms, for each 16-momentum tile,
+//   D[p][q] += sum_{s<4} phase[p][s] * R[s][q]
+// with v_mfma_f64_16x16x4f64: A = phases (lane: row p = lane&15, k = site
+// lane>>4), B = R (lane: k = site lane>>4, column q = lane&15) -- exactly the
+// registers the contraction leaves in each lane.  Complex products take four
+// real MFMAs.  Partial sums go to partial[item][p][16]; project_finish sums
+// them per timeslice with Neumaier compensation in a fixed order.
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void site_coords(size_t s, const int (&dims)[4], int order, int &x, int &y, int &z, int &t)
+{
+  if (order == 0) {
+    x = (int)(s % dims[0]);
+    s /= dims[0];
+    y = (int)(s % dims[1]);
+    s /= dims[1];
+    z = (int)(s % dims[2]);
+    t = (int)(s / dims[2]);
+  } else {
+    const size_t half = (size_t)dims[0] * dims[1] * dims[2] * dims[3] / 2;
+    const int parity = (int)(s / half);
+    size_t r = s - (size_t)parity * half;
+    const int lxh = dims[0] >> 1;
+    const int xh = (int)(r % lxh);
+    r /= lxh;
+    y = (int)(r % dims[1]);
+    r /= dims[1];
+    z = (int)(r % dims[2]);
+    t = (int)(r / dims[2]);
+    x = 2 * xh + ((parity + y + z + t) & 1);
+  }
+}
+
+__device__ __forceinline__ double2 momentum_phase(const ProjectArgs &pa, int p, int x, int y, int z)
+{
+  if (p >= pa.np) return make_double2(0.0, 0.0);
+  int kx = (pa.mom[p][0] * x) % pa.dims[0];
+  int ky = (pa.mom[p][1] * y) % pa.dims[1];
+  int kz = (pa.mom[p][2] * z) % pa.dims[2];
+  kx += kx < 0 ? pa.dims[0] : 0;
+  ky += ky < 0 ? pa.dims[1] : 0;
+  kz += kz < 0 ? pa.dims[2] : 0;
+  const double2 wx = pa.phase_tab[kx];
+  const double2 wy = pa.phase_tab[pa.dims[0] + ky];
+  const double2 wz = pa.phase_tab[pa.dims[0] + pa.dims[1] + kz];
+  return cmul(cmul(wx, wy), wz);
+}
+
+template <int NT>
+__device__ __forceinline__ void project_accumulate(dbl4 (&acc_re)[NT], dbl4 (&acc_im)[NT], const ProjectArgs &pa,
+                                                   double2 R, size_t site, int q)
+{
+  int x, y, z, t;
+  site_coords(site, pa.dims, pa.order, x, y, z, t);
+#pragma unroll
+  for (int tt = 0; tt < NT; ++tt) {
+    const double2 ph = momentum_phase(pa, tt * 16 + q, x, y, z);
+    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.x, R.x, acc_re[tt], 0, 0, 0);
+    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ph.y, R.y, acc_re[tt], 0, 0, 0);
+    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.x, R.y, acc_im[tt], 0, 0, 0);
+    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.y, R.x, acc_im[tt], 0, 0, 0);
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void project_store(const dbl4 (&acc_re)[NT], const dbl4 (&acc_im)[NT],
+                                              const ProjectArgs &pa, size_t item, int lane)
+{
+  const int g = lane >> 4, q = lane & 15;
+  double2 *dst = pa.partial + item * (size_t)pa.np * 16;
+#pragma unroll
+  for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int p = tt * 16 + g * 4 + r;
+      if (p < pa.np) dst[p * 16 + q] = make_double2(acc_re[tt][r], acc_im[tt][r]);
+    }
+}
+
+__device__ __forceinline__ void item_range(const ProjectArgs &pa, size_t item, size_t &first, size_t &last)
+{
+  const size_t slice = item / pa.chunks_per_slice;
+  const size_t c = item - slice * pa.chunks_per_slice;
+  const size_t s0 = slice * pa.slice_sites;
+  first = s0 + c * (size_t)pa.chunk;
+  last = first + pa.chunk;
+  const size_t s1 = s0 + pa.slice_sites;
+  last = last < s1 ? last : s1;
+  first = first < s1 ? first : s1;
+}
+
+// FP32 storage: rescale every staged site of every propagator by an exact
+// power of two so its largest entry lies in [1, 2).  The contraction is
+// trilinear, so the FP32 result times 2^(e_i + e_j + e_n) is the true value;
+// the product of three steeply decaying propagators (~1e-66 at the far end of
+// a 64-timeslice lattice) therefore never underflows FP32.
+__device__ __forceinline__ int normalise_site(float2 *site_lds, int q)
+{
+  float2 *p = site_lds + q * 9;
+  float mx = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) mx = fmaxf(mx, fmaxf(fabsf(p[e].x), fabsf(p[e].y)));
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m, 64));
+  int ex = (mx > 0.0f) ? ilogbf(mx) : 0;
+  ex = ex < -126 ? -126 : (ex > 126 ? 126 : ex);
+  const float sc = ldexpf(1.0f, -ex);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) p[e] = make_float2(p[e].x * sc, p[e].y * sc);
+  return ex;
+}
+
+template <int NT, typename Real>
+__global__ __launch_bounds__(WAVE) void baryon_project_kernel(BaryonProjectArgs a, ProjectArgs pa)
+{
+  using C = cplx<Real>;
+  __shared__ C lds[3][SITES_PER_WAVE * SITE_LDS];
+  const int lane = threadIdx.x;
+  const int g = lane >> 4, q = lane & 15;
+  const size_t item = blockIdx.x;
+  size_t first, last;
+  item_range(pa, item, first, last);
+
+  dbl4 acc_re[NT], acc_im[NT];
+#pragma unroll
+  for (int tt = 0; tt < NT; ++tt) acc_re[tt] = acc_im[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+  for (size_t s = first; s < last; s += SITES_PER_WAVE) {
+    const size_t left = last - s;
+    const int nsites = left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
+    stage_sites(lds[0], (const C *)a.propag_i, s, nsites, lane);
+    stage_sites(lds[1], (const C *)a.propag_j, s, nsites, lane);
+    stage_sites(lds[2], (const C *)a.propag_n, s, nsites, lane);
+    __syncthreads();
+    C *si = lds[0] + g * SITE_LDS, *sj = lds[1] + g * SITE_LDS, *sn = lds[2] + g * SITE_LDS;
+    int ex = 0;
+    if constexpr (sizeof(Real) == 4) {
+      ex = normalise_site(si, q) + normalise_site(sj, q) + normalise_site(sn, q);
+      __syncthreads();
+    }
+    double2 R = make_double2(0.0, 0.0);
+    for (int t = 0; t < a.nterm; ++t) {
+      const C r = baryon_open_rt<C>(a.types[t], si, sj, sn, q, a.gij, a.gklT);
+      R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)r.x, (double)r.y));
+    }
+    if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
+    const size_t site = (g < nsites) ? s + g : s; // padded lanes carry R = 0
+    project_accumulate<NT>(acc_re, acc_im, pa, R, site, q);
+    __syncthreads();
+  }
+  project_store<NT>(acc_re, acc_im, pa, item, lane);
+}
+
+template <typename Real>
+static void launch_project_t(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+{
+  const int nt = (pa.np + 15) / 16;
+  const dim3 grid((unsigned)items), block(WAVE);
+  switch (nt) {
+  case 1: hipLaunchKernelGGL((baryon_project_kernel<1, Real>), grid, block, 0, stream, a, pa); break;
+  case 2: hipLaunchKernelGGL((baryon_project_kernel<2, Real>), grid, block, 0, stream, a, pa); break;
+  case 3: hipLaunchKernelGGL((baryon_project_kernel<3, Real>), grid, block, 0, stream, a, pa); break;
+  default: hipLaunchKernelGGL((baryon_project_kernel<4, Real>), grid, block, 0, stream, a, pa); break;
+  }
+}
+
+void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+{
+  launch_project_t<double>(a, pa, items, stream);
+}
+
+void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+{
+  launch_project_t<float>(a, pa, items, stream);
+}
+
+} // namespace contract_amd

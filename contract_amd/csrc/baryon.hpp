@@ -1,0 +1,108 @@
+// baryon.hpp -- kernel argument blocks and launchers of every kernel family.
+// Arguments travel by value in the kernel-argument segment; nothing is copied
+// to __constant__ memory, so launches are asynchronous and thread-safe.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "device.hpp"
+
+namespace contract_amd
+{
+
+constexpr int MAX_TERMS = 6;
+constexpr int MAX_MOM = 64;
+constexpr int MAX_PROJ_PER_LAUNCH = 8;
+
+// ---- baryon, drop-in per-site form -------------------------------------
+struct BaryonSiteArgs {
+  const void *propag_i, *propag_j, *propag_n;
+  void *correl;
+  size_t volume;
+  SparseSpin gij;  // Gij: row i -> (j, Gij[i,j])
+  SparseSpin gklT; // Gkl^T: row l -> (k, Gkl[k,l])
+  double w_re[16], w_im[16]; // projector W[m][n] at q = m*4 + n
+};
+void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream);
+
+// ---- timeslice / momentum projection shared by the fused kernels --------
+struct ProjectArgs {
+  int dims[4]; // Lx, Ly, Lz, Lt
+  int order;   // 0 lexicographic [t][z][y][x]; 1 even-odd [parity][t][z][y][x/2]
+  int np;
+  int mom[MAX_MOM][3];
+  const double2 *phase_tab; // exp(-2 pi i k / L_mu), [Lx | Ly | Lz]
+  int chunk;                // sites per work item (multiple of 4)
+  int chunks_per_slice;
+  size_t slice_sites;       // sites per (parity, t) slice
+  double2 *partial;         // [items][np][16]
+};
+
+struct BaryonProjectArgs {
+  const void *propag_i, *propag_j, *propag_n;
+  SparseSpin gij, gklT;
+  int nterm;
+  int types[MAX_TERMS];
+  double w_re[MAX_TERMS], w_im[MAX_TERMS];
+};
+void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream);
+// Same with complex64 propagators: FP32 per-site contraction on blocks rescaled
+// by exact powers of two, FP64 projection and sums.
+void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream);
+
+// partial[items][np][16] -> out[w][p][t] = sum_q W_w[q] * sum_{items of t} partial
+struct FinishArgs {
+  const double2 *partial;
+  int np, lt, order;
+  int chunks_per_slice;
+  int nproj;      // projectors in this launch (<= MAX_PROJ_PER_LAUNCH)
+  int proj_base;  // index of the first one in the output
+  double w_re[MAX_PROJ_PER_LAUNCH][16], w_im[MAX_PROJ_PER_LAUNCH][16];
+  double2 *out;   // [nproj_total][np][lt]
+};
+void launch_project_finish(const FinishArgs &f, hipStream_t stream);
+void launch_phase_table(double2 *tab, const int dims[3], hipStream_t stream);
+
+// ---- meson --------------------------------------------------------------
+struct MesonArgs {
+  const void *propag_i, *propag_j;
+  size_t volume;
+  SparseSpin a; // (g5 Gij)^T : row i -> (j, (g5 Gij)[j,i])
+  SparseSpin b; // (g5 Gkl)^T : row l -> (k, (g5 Gkl)[k,l])
+  void *correl;
+};
+void launch_meson(const MesonArgs &a, hipStream_t stream);
+
+// 16 correlators of one fixed and one running insertion (meson_all_*).
+struct MesonAllArgs {
+  const void *propag_i, *propag_j;
+  size_t volume;
+  int sink;              // 0: source form (Gij fixed), 1: sink form (Gkl fixed)
+  SparseSpin fixed;      // the fixed (g5 G)^T
+  SparseSpin run[16];    // (g5 G(g))^T for g = 0..15
+  void *correl[16];
+};
+void launch_meson_all(const MesonAllArgs &a, hipStream_t stream);
+
+// ---- diquark ------------------------------------------------------------
+struct DiquarkArgs {
+  const void *propag_i, *propag_j;
+  void *diquark;
+  size_t volume;
+  SparseSpin gij, gklT;
+};
+void launch_diquark(const DiquarkArgs &a, hipStream_t stream);
+
+// ---- sequential ---------------------------------------------------------
+struct SequentialArgs {
+  const void *in0, *in1; // the two propagators that stay, in (i, j, n) order
+  void *out;             // buffer of the removed propagator (never read)
+  size_t volume;
+  // each spin matrix in both orientations: G (row r -> column, G[r][c]) and
+  // G^T (row c -> r, G[r][c])
+  SparseSpin gij, gijT, gkl, gklT, gmn, gmnT;
+};
+void launch_sequential(int type, int target, const SequentialArgs &a, hipStream_t stream);
+
+} // namespace contract_amd

@@ -1,0 +1,260 @@
+// device.hpp -- complex arithmetic, lane collectives and HBM->LDS staging for
+// the gfx950 contraction kernels.
+//
+// Work mapping shared by every per-site kernel: a site is handled by 16 lanes
+// of a wave64 (lane q = lane & 15 indexes a 4x4 spin pair), so one wave covers
+// SITES_PER_WAVE = 4 sites.  The propagators of those sites are staged into
+// LDS with fully coalesced 16-byte loads (a wave reads 1 KiB contiguous per
+// instruction), then each lane reads the 3x3 colour blocks it needs into
+// registers and works from there.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace contract_amd
+{
+
+constexpr int NS = 4;
+constexpr int NC = 3;
+constexpr int SITE_ELEMS = 144; // [4][4][3][3] complex entries per site
+constexpr int LANES_PER_SITE = 16;
+constexpr int WAVE = 64;
+constexpr int SITES_PER_WAVE = WAVE / LANES_PER_SITE;
+
+// ---------------------------------------------------------------------------
+// complex helpers (double2 / float2 hold (re, im))
+// ---------------------------------------------------------------------------
+template <typename R> struct cplx_t;
+template <> struct cplx_t<double> {
+  using type = double2;
+};
+template <> struct cplx_t<float> {
+  using type = float2;
+};
+template <typename R> using cplx = typename cplx_t<R>::type;
+
+template <typename C> __device__ __forceinline__ C cmake(decltype(C::x) re, decltype(C::x) im)
+{
+  C z;
+  z.x = re;
+  z.y = im;
+  return z;
+}
+template <typename C> __device__ __forceinline__ C czero() { return cmake<C>(0, 0); }
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmake<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmake<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+  return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// acc + a * b
+template <typename C> __device__ __forceinline__ C cfma(C acc, C a, C b)
+{
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+  return acc;
+}
+// acc - a * b
+template <typename C> __device__ __forceinline__ C cfms(C acc, C a, C b)
+{
+  acc.x = fma(-a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(-a.x, b.y, acc.y);
+  acc.y = fma(-a.y, b.x, acc.y);
+  return acc;
+}
+// acc + a * conj(b)
+template <typename C> __device__ __forceinline__ C cfma_conj(C acc, C a, C b)
+{
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(a.y, b.x, acc.y);
+  acc.y = fma(-a.x, b.y, acc.y);
+  return acc;
+}
+template <typename C> __device__ __forceinline__ C cconj(C a) { return cmake<C>(a.x, -a.y); }
+template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) { return cmake<C>(a.x * s, a.y * s); }
+
+// ---------------------------------------------------------------------------
+// lane collectives inside a 16-lane site group
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }
+__device__ __forceinline__ float shfl_xor_d(float v, int mask) { return __shfl_xor(v, mask, 64); }
+
+template <typename C> __device__ __forceinline__ C shfl_xor_c(C v, int mask)
+{
+  return cmake<C>(shfl_xor_d(v.x, mask), shfl_xor_d(v.y, mask));
+}
+
+// Sum over the 16 lanes of a site group; every lane of the group gets the sum.
+template <typename C> __device__ __forceinline__ C group16_allreduce(C v)
+{
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1) v = cadd(v, shfl_xor_c(v, m));
+  return v;
+}
+
+// Sum over the lanes that differ only in the bits of `span` (e.g. 12 = bits 2,3).
+template <typename C, int SPAN> __device__ __forceinline__ C group_allreduce_bits(C v)
+{
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1)
+    if (SPAN & m) v = cadd(v, shfl_xor_c(v, m));
+  return v;
+}
+
+// Reduce-scatter over a 16-lane group: every lane holds v[0..15]; on return
+// lane q holds sum over the group of v[q].  Fifteen exchanges instead of the
+// 4 x 16 of sixteen separate all-reduces.
+template <typename C> __device__ __forceinline__ C group16_reduce_scatter(C (&v)[16], int q)
+{
+  C a8[8];
+  {
+    const bool hi = q & 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      C keep = hi ? v[k + 8] : v[k];
+      C send = hi ? v[k] : v[k + 8];
+      a8[k] = cadd(keep, shfl_xor_c(send, 8));
+    }
+  }
+  C a4[4];
+  {
+    const bool hi = q & 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      C keep = hi ? a8[k + 4] : a8[k];
+      C send = hi ? a8[k] : a8[k + 4];
+      a4[k] = cadd(keep, shfl_xor_c(send, 4));
+    }
+  }
+  C a2[2];
+  {
+    const bool hi = q & 2;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      C keep = hi ? a4[k + 2] : a4[k];
+      C send = hi ? a4[k] : a4[k + 2];
+      a2[k] = cadd(keep, shfl_xor_c(send, 2));
+    }
+  }
+  const bool hi = q & 1;
+  C keep = hi ? a2[1] : a2[0];
+  C send = hi ? a2[0] : a2[1];
+  return cadd(keep, shfl_xor_c(send, 1));
+}
+
+// ---------------------------------------------------------------------------
+// HBM -> LDS staging of SITES_PER_WAVE consecutive sites of one propagator.
+// Source layout is the ABI's site-major [V][144] complex.  The LDS copy keeps
+// the same order with a site stride of SITE_LDS complex entries (144 + pad).
+// Sites beyond `nsites` (ragged tail) are zero-filled so the arithmetic on
+// them is harmless; their results are never stored.
+// ---------------------------------------------------------------------------
+constexpr int SITE_LDS = SITE_ELEMS + 1; // pad one complex: successive sites start on different banks
+
+template <typename C>
+__device__ __forceinline__ void stage_sites(C *__restrict__ lds, const C *__restrict__ src, size_t first_site,
+                                            int nsites, int lane)
+{
+  const C *base = src + first_site * SITE_ELEMS;
+  const int total = nsites * SITE_ELEMS;
+#pragma unroll
+  for (int it = 0; it < SITES_PER_WAVE * SITE_ELEMS / WAVE; ++it) {
+    const int e = it * WAVE + lane; // element within the 4-site chunk
+    const int s = e / SITE_ELEMS;
+    const int r = e - s * SITE_ELEMS;
+    C v = (e < total) ? base[e] : czero<C>();
+    lds[s * SITE_LDS + r] = v;
+  }
+}
+
+// LDS -> HBM: the inverse of stage_sites, for propagator-shaped outputs.
+template <typename C>
+__device__ __forceinline__ void unstage_sites(C *__restrict__ dst, const C *__restrict__ lds, size_t first_site,
+                                              int nsites, int lane)
+{
+  C *base = dst + first_site * SITE_ELEMS;
+  const int total = nsites * SITE_ELEMS;
+#pragma unroll
+  for (int it = 0; it < SITES_PER_WAVE * SITE_ELEMS / WAVE; ++it) {
+    const int e = it * WAVE + lane;
+    const int s = e / SITE_ELEMS;
+    const int r = e - s * SITE_ELEMS;
+    if (e < total) base[e] = lds[s * SITE_LDS + r];
+  }
+}
+
+// Read one 3x3 colour block [s1][s2] of a staged site into registers.
+template <typename C> __device__ __forceinline__ void load_block(C (&b)[9], const C *__restrict__ site_lds, int s1, int s2)
+{
+  const C *p = site_lds + (s1 * 4 + s2) * 9;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) b[e] = p[e];
+}
+
+// ---------------------------------------------------------------------------
+// colour algebra
+// ---------------------------------------------------------------------------
+// cof(Y,Z)[a,d] = sum_{bc,ef} eps_abc eps_def Y[b,e] Z[c,f], accumulated with
+// weight w:  acc[a,d] += w * cof(Y,Z)[a,d].  Symmetric in Y and Z; the triple
+// product ee(X,Y,Z) = sum_ad X[a,d] cof(Y,Z)[a,d].
+template <typename C> __device__ __forceinline__ void cof_acc(C (&acc)[9], const C (&Y)[9], const C (&Z)[9], C w)
+{
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int e = (d + 1) % 3, f = (d + 2) % 3;
+      C t = cmul(Y[b * 3 + e], Z[c * 3 + f]);
+      t = cfma(t, Y[c * 3 + f], Z[b * 3 + e]);
+      t = cfms(t, Y[b * 3 + f], Z[c * 3 + e]);
+      t = cfms(t, Y[c * 3 + e], Z[b * 3 + f]);
+      acc[a * 3 + d] = cfma(acc[a * 3 + d], w, t);
+    }
+  }
+}
+
+template <typename C> __device__ __forceinline__ C dot9(const C (&X)[9], const C (&Y)[9])
+{
+  C s = cmul(X[0], Y[0]);
+#pragma unroll
+  for (int e = 1; e < 9; ++e) s = cfma(s, X[e], Y[e]);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// sparse spin matrices: every Gamma in the DeGrand-Rossi basis (and gamma_5
+// times one) has exactly one nonzero per row, so a 4x4 matrix is stored as
+// the column of that entry per row and its value.
+// ---------------------------------------------------------------------------
+struct SparseSpin {
+  int col[4];
+  double re[4];
+  double im[4];
+};
+
+// Row lookups with a lane-dependent row go through selects, so the argument
+// block stays in scalar registers instead of being copied to scratch for
+// indexing.
+template <typename T> __device__ __forceinline__ T sel4(const T (&a)[4], int i)
+{
+  T v = a[0];
+  v = (i == 1) ? a[1] : v;
+  v = (i == 2) ? a[2] : v;
+  v = (i == 3) ? a[3] : v;
+  return v;
+}
+
+__device__ __forceinline__ int sparse_col(const SparseSpin &g, int row) { return sel4(g.col, row); }
+
+template <typename C> __device__ __forceinline__ C sparse_val(const SparseSpin &g, int row)
+{
+  return cmake<C>((decltype(C::x))sel4(g.re, row), (decltype(C::x))sel4(g.im, row));
+}
+
+} // namespace contract_amd

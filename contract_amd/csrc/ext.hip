@@ -1,0 +1,264 @@
+// ext.hip -- extension API (include/contract_ext.h): fused contraction +
+// timeslice/momentum projection, error reporting, workspace planning.
+
+#include <contract_ext.h>
+
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "baryon.hpp"
+#include "plan.hpp"
+#include "runtime.hpp"
+#include "spin.hpp"
+
+namespace contract_amd
+{
+
+static thread_local char g_last_error[512] = "";
+static thread_local hipEvent_t g_timer_start = nullptr, g_timer_stop = nullptr;
+
+hipError_t timer_begin(hipStream_t stream)
+{
+  return g_timer_start ? hipEventRecord(g_timer_start, stream) : hipSuccess;
+}
+hipError_t timer_end(hipStream_t stream) { return g_timer_stop ? hipEventRecord(g_timer_stop, stream) : hipSuccess; }
+
+void set_last_error(const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+int device_cu_count()
+{
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+  return n;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item)
+{
+  ProjectPlan p{};
+  const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  p.slices = dims[3] * (order ? 2 : 1);
+  p.slice_sites = V / p.slices;
+  size_t chunk;
+  if (sites_per_item > 0) {
+    chunk = (size_t)sites_per_item;
+  } else {
+    // enough work items to fill the chip several times over
+    const size_t target_items = (size_t)device_cu_count() * 16;
+    size_t cps = (target_items + p.slices - 1) / p.slices;
+    if (cps < 1) cps = 1;
+    chunk = (p.slice_sites + cps - 1) / cps;
+  }
+  chunk = (chunk + SITES_PER_WAVE - 1) / SITES_PER_WAVE * SITES_PER_WAVE;
+  if (chunk < SITES_PER_WAVE) chunk = SITES_PER_WAVE;
+  p.chunk = (int)chunk;
+  p.chunks_per_slice = (int)((p.slice_sites + chunk - 1) / chunk);
+  p.items = (size_t)p.slices * p.chunks_per_slice;
+  p.tab_bytes = align256(sizeof(double2) * (dims[0] + dims[1] + dims[2]));
+  p.partial_bytes = align256(p.items * (size_t)nmom * 16 * sizeof(double2));
+  p.total = p.tab_bytes + p.partial_bytes;
+  return p;
+}
+
+int check_lattice(const int dims[4], int order)
+{
+  if (!dims) {
+    set_last_error("dims is NULL");
+    return CONTRACT_ERR_ARG;
+  }
+  for (int mu = 0; mu < 4; ++mu)
+    if (dims[mu] <= 0 || dims[mu] > 4096) {
+      set_last_error("dims[%d] = %d out of range", mu, dims[mu]);
+      return CONTRACT_ERR_ARG;
+    }
+  if (order != 0 && order != 1) {
+    set_last_error("site_order %d is not 0 (lexicographic) or 1 (even-odd)", order);
+    return CONTRACT_ERR_ARG;
+  }
+  if (order == 1 && (dims[0] % 2)) {
+    set_last_error("even-odd order needs an even Lx, got %d", dims[0]);
+    return CONTRACT_ERR_ARG;
+  }
+  return CONTRACT_OK;
+}
+
+int check_momenta(const int *momenta, int nmom)
+{
+  if (nmom < 1 || nmom > MAX_MOM || !momenta) {
+    set_last_error("nmom = %d must be in 1..%d with a non-NULL momenta array", nmom, MAX_MOM);
+    return CONTRACT_ERR_ARG;
+  }
+  return CONTRACT_OK;
+}
+
+void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const int dims[4], int order, const int *momenta,
+                       int nmom, void *ws)
+{
+  std::memset(&pa, 0, sizeof(pa));
+  for (int mu = 0; mu < 4; ++mu) pa.dims[mu] = dims[mu];
+  pa.order = order;
+  pa.np = nmom;
+  for (int p = 0; p < nmom; ++p)
+    for (int mu = 0; mu < 3; ++mu) pa.mom[p][mu] = momenta[3 * p + mu];
+  pa.phase_tab = reinterpret_cast<const double2 *>(ws);
+  pa.chunk = plan.chunk;
+  pa.chunks_per_slice = plan.chunks_per_slice;
+  pa.slice_sites = plan.slice_sites;
+  pa.partial = reinterpret_cast<double2 *>(static_cast<char *>(ws) + plan.tab_bytes);
+}
+
+// Launch the finishing reductions for nproj projectors (8 per launch).
+void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const zhost *W, int nproj, void *out,
+                       hipStream_t stream)
+{
+  for (int base = 0; base < nproj; base += MAX_PROJ_PER_LAUNCH) {
+    FinishArgs f;
+    std::memset(&f, 0, sizeof(f));
+    f.partial = pa.partial;
+    f.np = pa.np;
+    f.lt = pa.dims[3];
+    f.order = pa.order;
+    f.chunks_per_slice = plan.chunks_per_slice;
+    f.nproj = nproj - base < MAX_PROJ_PER_LAUNCH ? nproj - base : MAX_PROJ_PER_LAUNCH;
+    f.proj_base = base;
+    for (int w = 0; w < f.nproj; ++w)
+      for (int q = 0; q < 16; ++q) {
+        f.w_re[w][q] = W[(base + w) * 16 + q].real();
+        f.w_im[w][q] = W[(base + w) * 16 + q].imag();
+      }
+    f.out = static_cast<double2 *>(out);
+    launch_project_finish(f, stream);
+  }
+}
+
+} // namespace contract_amd
+
+using namespace contract_amd;
+
+extern "C" {
+
+const char *contract_last_error(void) { return g_last_error; }
+
+int contract_set_kernel_timer(void *start, void *stop)
+{
+  g_timer_start = static_cast<hipEvent_t>(start);
+  g_timer_stop = static_cast<hipEvent_t>(stop);
+  return CONTRACT_OK;
+}
+
+int contract_gamma_matrix(int g, double _Complex *out)
+{
+  if (g < 0 || g > 15 || !out) {
+    set_last_error("gamma index %d out of range 0..15", g);
+    return CONTRACT_ERR_ARG;
+  }
+  const Mat4 G = gamma_matrix(g);
+  double *o = reinterpret_cast<double *>(out);
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      o[2 * (r * 4 + c)] = G.v[r][c].real();
+      o[2 * (r * 4 + c) + 1] = G.v[r][c].imag();
+    }
+  return CONTRACT_OK;
+}
+
+size_t contract_project_workspace_bytes(const int dims[4], int site_order, int nmom)
+{
+  if (check_lattice(dims, site_order) != CONTRACT_OK || nmom < 1 || nmom > MAX_MOM) return 0;
+  return plan_project(dims, site_order, nmom, 0).total;
+}
+
+int contract_baryon_twopoint_project(void *out, const void *propag_i, const void *propag_j, const void *propag_n,
+                                     int precision, const int *contract_types, const double _Complex *weights,
+                                     int nterm, int gamma_ij, int gamma_kl, const double _Complex *projectors,
+                                     int nproj, const int *momenta, int nmom, const int dims[4], int site_order,
+                                     void *workspace, size_t workspace_bytes, void *stream_)
+{
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  int rc;
+  if ((rc = check_lattice(dims, site_order)) != CONTRACT_OK) return rc;
+  if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
+  if (!out || !propag_i || !propag_j || !propag_n) {
+    set_last_error("NULL output or propagator pointer");
+    return CONTRACT_ERR_ARG;
+  }
+  if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32) {
+    set_last_error("precision %d is not CONTRACT_FP64 or CONTRACT_FP32", precision);
+    return CONTRACT_ERR_ARG;
+  }
+  if (nterm < 1 || nterm > MAX_TERMS || !contract_types || !weights) {
+    set_last_error("nterm = %d must be in 1..%d", nterm, MAX_TERMS);
+    return CONTRACT_ERR_ARG;
+  }
+  for (int t = 0; t < nterm; ++t)
+    if (contract_types[t] < 0 || contract_types[t] > 5) {
+      set_last_error("contract_types[%d] = %d is not a BaryonContractType", t, contract_types[t]);
+      return CONTRACT_ERR_ARG;
+    }
+  if (gamma_ij < 0 || gamma_ij > 15 || gamma_kl < 0 || gamma_kl > 15) {
+    set_last_error("gamma_ij = %d / gamma_kl = %d out of range 0..15", gamma_ij, gamma_kl);
+    return CONTRACT_ERR_ARG;
+  }
+  if (nproj < 1 || !projectors) {
+    set_last_error("nproj = %d must be >= 1 with a non-NULL projectors array", nproj);
+    return CONTRACT_ERR_ARG;
+  }
+
+  const ProjectPlan plan = plan_project(dims, site_order, nmom, 0);
+  void *ws = workspace;
+  bool own = false;
+  if (ws) {
+    if (workspace_bytes < plan.total) {
+      set_last_error("workspace of %zu bytes is smaller than the %zu needed", workspace_bytes, plan.total);
+      return CONTRACT_ERR_WORKSPACE;
+    }
+  } else {
+    CONTRACT_EXT_HIP(hipMallocAsync(&ws, plan.total, stream));
+    own = true;
+  }
+
+  ProjectArgs pa;
+  fill_project_args(pa, plan, dims, site_order, momenta, nmom, ws);
+  launch_phase_table(const_cast<double2 *>(pa.phase_tab), dims, stream);
+
+  BaryonProjectArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.propag_i = propag_i;
+  a.propag_j = propag_j;
+  a.propag_n = propag_n;
+  a.gij = sparse_of(gamma_matrix(gamma_ij));
+  a.gklT = sparse_of(mat_transpose(gamma_matrix(gamma_kl)));
+  a.nterm = nterm;
+  const double *wt = reinterpret_cast<const double *>(weights);
+  for (int t = 0; t < nterm; ++t) {
+    a.types[t] = contract_types[t];
+    a.w_re[t] = wt[2 * t];
+    a.w_im[t] = wt[2 * t + 1];
+  }
+  CONTRACT_EXT_HIP(timer_begin(stream));
+  if (precision == CONTRACT_FP64)
+    launch_baryon_project(a, pa, plan.items, stream);
+  else
+    launch_baryon_project_fp32(a, pa, plan.items, stream);
+  CONTRACT_EXT_HIP(hipGetLastError());
+  CONTRACT_EXT_HIP(timer_end(stream));
+
+  std::vector<zhost> W((size_t)nproj * 16);
+  const double *pm = reinterpret_cast<const double *>(projectors);
+  for (size_t e = 0; e < W.size(); ++e) W[e] = zhost(pm[2 * e], pm[2 * e + 1]);
+  launch_finish_all(pa, plan, W.data(), nproj, out, stream);
+  CONTRACT_EXT_HIP(hipGetLastError());
+  if (own) CONTRACT_EXT_HIP(hipFreeAsync(ws, stream));
+  return CONTRACT_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,256 @@
+"""Python face of libcontract.so, shaped like the reference's plugin binding.
+
+The reference is consumed from Python through a module that
+``python -m pyquda_plugins -i contract.h -l contract`` generates from
+include/contract.h (README.md:13); PyQUDA scripts call it as
+``pycontract.baryonTwoPoint(...)`` (baryon_2pt.py:92,108).  This module offers
+the same surface over the gfx950 library:
+
+* the C entry points under their C names with the C argument order
+  (``meson_two_point(correl, propag_i, propag_j, volume, gamma_ij, gamma_kl)``
+  ...), accepting device pointers as ints or torch tensors;
+* the ``BaryonContractType`` / ``BaryonSequentialType`` enums with every
+  alias of include/contract_enum.h;
+* tensor-level helpers (``mesonTwoPoint``, ``baryonTwoPoint``, ...) that take
+  torch complex128 propagators ``[V, 4, 4, 3, 3]`` on the GPU and return the
+  correlator, plus ``baryonTwoPointProject`` for the fused timeslice/momentum
+  extension.
+
+Launches go to the null stream, like the reference's; synchronise before
+reading results on the host.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import enum
+from typing import Sequence
+
+import numpy as np
+
+from ._lib import last_error, lib
+
+
+class BaryonContractType(enum.IntEnum):
+    IK_JL_NM = 0
+    IK_JM_NL = 1
+    IL_JK_NM = 2
+    IL_JM_NK = 3
+    IM_JK_NL = 4
+    IM_JL_NK = 5
+    AD_BE_CF = 0
+    AD_BF_CE = 1
+    AE_BD_CF = 2
+    AE_BF_CD = 3
+    AF_BD_CE = 4
+    AF_BE_CD = 5
+    IK_JL_MN = 0
+    IK_JN_ML = 1
+    IL_JK_MN = 2
+    IL_JN_MK = 3
+    IN_JK_ML = 4
+    IN_JL_MK = 5
+
+
+class BaryonSequentialType(enum.IntEnum):
+    SEQUENTIAL_I = 0
+    SEQUENTIAL_J = 1
+    SEQUENTIAL_N = 2
+
+
+CONTRACT_FP64 = 0
+CONTRACT_FP32 = 1
+LEXICOGRAPHIC = 0
+EVEN_ODD = 1
+
+
+def _ptr(x) -> int:
+    """Device pointer of a torch tensor, or an int passed through."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    raise TypeError(f"expected a device pointer or tensor, got {type(x)}")
+
+
+# ---------------------------------------------------------------------------
+# C entry points (include/contract.h)
+# ---------------------------------------------------------------------------
+
+def init(device: int) -> None:
+    lib().init(int(device))
+
+
+def meson_two_point(correl, propag_i, propag_j, volume: int, gamma_ij: int, gamma_kl: int) -> None:
+    lib().meson_two_point(_ptr(correl), _ptr(propag_i), _ptr(propag_j), int(volume), int(gamma_ij), int(gamma_kl))
+
+
+def _ptr_array(correl) -> ctypes.Array:
+    ptrs = [_ptr(c) for c in correl]
+    if len(ptrs) != 16:
+        raise ValueError("meson_all_* need 16 output pointers")
+    return (ctypes.c_void_p * 16)(*ptrs)
+
+
+def meson_all_source_two_point(correl, propag_i, propag_j, volume: int, gamma_ij: int) -> None:
+    lib().meson_all_source_two_point(_ptr_array(correl), _ptr(propag_i), _ptr(propag_j), int(volume), int(gamma_ij))
+
+
+def meson_all_sink_two_point(correl, propag_i, propag_j, volume: int, gamma_kl: int) -> None:
+    lib().meson_all_sink_two_point(_ptr_array(correl), _ptr(propag_i), _ptr(propag_j), int(volume), int(gamma_kl))
+
+
+def baryon_diquark(diquark, propag_i, propag_j, volume: int, gamma_ij: int, gamma_kl: int) -> None:
+    lib().baryon_diquark(_ptr(diquark), _ptr(propag_i), _ptr(propag_j), int(volume), int(gamma_ij), int(gamma_kl))
+
+
+def baryon_two_point(correl, propag_i, propag_j, propag_n, contract_type, volume: int, gamma_ij: int,
+                     gamma_kl: int, gamma_mn: int) -> None:
+    lib().baryon_two_point(_ptr(correl), _ptr(propag_i), _ptr(propag_j), _ptr(propag_n), int(contract_type),
+                           int(volume), int(gamma_ij), int(gamma_kl), int(gamma_mn))
+
+
+def baryon_general_two_point(correl, propag_i, propag_j, propag_n, contract_type, volume: int, gamma_ij: int,
+                             gamma_kl: int, project_mn) -> None:
+    proj = np.ascontiguousarray(np.asarray(project_mn, dtype=np.complex128).reshape(16))
+    lib().baryon_general_two_point(_ptr(correl), _ptr(propag_i), _ptr(propag_j), _ptr(propag_n),
+                                   int(contract_type), int(volume), int(gamma_ij), int(gamma_kl),
+                                   proj.ctypes.data_as(ctypes.c_void_p))
+
+
+def baryon_sequential_two_point(propag_i, propag_j, propag_n, contract_type, sequential_type, volume: int,
+                                gamma_ij: int, gamma_kl: int, gamma_mn: int) -> None:
+    lib().baryon_sequential_two_point(_ptr(propag_i), _ptr(propag_j), _ptr(propag_n), int(contract_type),
+                                      int(sequential_type), int(volume), int(gamma_ij), int(gamma_kl),
+                                      int(gamma_mn))
+
+
+# ---------------------------------------------------------------------------
+# tensor-level helpers
+# ---------------------------------------------------------------------------
+
+def gamma_matrix(g: int) -> np.ndarray:
+    """Dense DeGrand-Rossi G(g) as computed by the library."""
+    out = np.zeros((4, 4), np.complex128)
+    if lib().contract_gamma_matrix(int(g), out.ctypes.data_as(ctypes.c_void_p)) != 0:
+        raise ValueError(last_error())
+    return out
+
+
+def _volume(p) -> int:
+    if p.dtype.__str__() not in ("torch.complex128", "torch.complex64"):
+        raise TypeError(f"propagators must be complex128/complex64 tensors, got {p.dtype}")
+    if not p.is_contiguous():
+        raise ValueError("propagators must be contiguous [V, 4, 4, 3, 3]")
+    if p.numel() % 144:
+        raise ValueError("propagator size is not a multiple of 144 complex entries")
+    return p.numel() // 144
+
+
+def mesonTwoPoint(propag_i, propag_j, gamma_ij: int, gamma_kl: int):
+    import torch
+    V = _volume(propag_i)
+    out = torch.empty(V, dtype=torch.complex128, device=propag_i.device)
+    meson_two_point(out, propag_i, propag_j, V, gamma_ij, gamma_kl)
+    return out
+
+
+def mesonAllSourceTwoPoint(propag_i, propag_j, gamma_ij: int):
+    import torch
+    V = _volume(propag_i)
+    out = torch.empty((16, V), dtype=torch.complex128, device=propag_i.device)
+    meson_all_source_two_point([out[g] for g in range(16)], propag_i, propag_j, V, gamma_ij)
+    return out
+
+
+def mesonAllSinkTwoPoint(propag_i, propag_j, gamma_kl: int):
+    import torch
+    V = _volume(propag_i)
+    out = torch.empty((16, V), dtype=torch.complex128, device=propag_i.device)
+    meson_all_sink_two_point([out[g] for g in range(16)], propag_i, propag_j, V, gamma_kl)
+    return out
+
+
+def baryonDiquark(propag_i, propag_j, gamma_ij: int, gamma_kl: int):
+    import torch
+    V = _volume(propag_i)
+    out = torch.empty((V, 4, 4, 3, 3), dtype=torch.complex128, device=propag_i.device)
+    baryon_diquark(out, propag_i, propag_j, V, gamma_ij, gamma_kl)
+    return out
+
+
+def baryonTwoPoint(propag_i, propag_j, propag_n, contract_type, gamma_ij: int, gamma_kl: int, gamma_mn):
+    """Per-site baryon correlator.  ``gamma_mn`` is a gamma index (sparse
+    path) or a 4x4 projector matrix, row-major [m][n] (general path)."""
+    import torch
+    V = _volume(propag_i)
+    out = torch.empty(V, dtype=torch.complex128, device=propag_i.device)
+    if isinstance(gamma_mn, (int, np.integer)):
+        baryon_two_point(out, propag_i, propag_j, propag_n, contract_type, V, gamma_ij, gamma_kl, gamma_mn)
+    else:
+        baryon_general_two_point(out, propag_i, propag_j, propag_n, contract_type, V, gamma_ij, gamma_kl,
+                                 np.asarray(gamma_mn))
+    return out
+
+
+def baryonSequentialTwoPoint(propag_i, propag_j, propag_n, contract_type, sequential_type, gamma_ij: int,
+                             gamma_kl: int, gamma_mn: int):
+    """Overwrites and returns the propagator selected by ``sequential_type``."""
+    V = _volume(propag_i)
+    baryon_sequential_two_point(propag_i, propag_j, propag_n, contract_type, sequential_type, V, gamma_ij,
+                                gamma_kl, gamma_mn)
+    return (propag_i, propag_j, propag_n)[int(sequential_type)]
+
+
+def _projectors(insertions) -> np.ndarray:
+    mats = []
+    for w in insertions:
+        if isinstance(w, (int, np.integer)):
+            mats.append(gamma_matrix(int(w)))
+        else:
+            mats.append(np.asarray(w, dtype=np.complex128).reshape(4, 4))
+    return np.ascontiguousarray(np.stack(mats))
+
+
+def project_workspace_bytes(dims: Sequence[int], nmom: int, site_order: int = LEXICOGRAPHIC) -> int:
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    return int(lib().contract_project_workspace_bytes(d.ctypes.data_as(ctypes.c_void_p), int(site_order),
+                                                      int(nmom)))
+
+
+def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, gamma_kl: int, insertions,
+                          momenta, dims, site_order: int = LEXICOGRAPHIC, out=None, workspace=None,
+                          stream=None):
+    """Fused baryon two-point: sum over ``terms`` = [(contract_type, weight)],
+    for every insertion (gamma index or 4x4 projector), projected to
+    ``out[len(insertions), len(momenta), Lt]`` (complex128, on the device).
+
+    Propagators may be complex128 (FP64 path) or complex64 (FP32 contraction,
+    FP64 accumulation)."""
+    import torch
+    precision = CONTRACT_FP32 if propag_i.dtype == torch.complex64 else CONTRACT_FP64
+    for p in (propag_i, propag_j, propag_n):
+        if p.dtype != propag_i.dtype:
+            raise TypeError("all three propagators must have the same dtype")
+        _volume(p)
+    W = _projectors(insertions)
+    mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
+    types = np.ascontiguousarray([int(t) for t, _ in terms], dtype=np.int32)
+    weights = np.ascontiguousarray([complex(w) for _, w in terms], dtype=np.complex128)
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    if out is None:
+        out = torch.empty((W.shape[0], mom.shape[0], int(d[3])), dtype=torch.complex128, device=propag_i.device)
+    ws_ptr, ws_bytes = 0, 0
+    if workspace is not None:
+        ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    vp = ctypes.c_void_p
+    rc = lib().contract_baryon_twopoint_project(
+        _ptr(out), _ptr(propag_i), _ptr(propag_j), _ptr(propag_n), precision, types.ctypes.data_as(vp),
+        weights.ctypes.data_as(vp), len(types), int(gamma_ij), int(gamma_kl), W.ctypes.data_as(vp), W.shape[0],
+        mom.ctypes.data_as(vp), mom.shape[0], d.ctypes.data_as(vp), int(site_order), ws_ptr, ws_bytes,
+        0 if stream is None else int(stream))
+    if rc != 0:
+        raise RuntimeError(f"contract_baryon_twopoint_project failed ({rc}): {last_error()}")
+    return out

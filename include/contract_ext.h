@@ -1,0 +1,78 @@
+/*
+ * contract_ext.h -- extension API of libcontract.so (no counterpart in the
+ * reference's include/contract.h, which stays unchanged).
+ *
+ * The reference stops at per-site correlators and leaves the timeslice sum
+ * and momentum projection to the caller (cupy in PyQUDA).  These entry points
+ * fuse them into the contraction: propagators are read from HBM once and the
+ * result is the projected correlator
+ *
+ *   out[w][p][t] = sum_{x in timeslice t} exp(-i 2pi sum_mu p_mu x_mu / L_mu)
+ *                  * sum_terms weight * C_{type}(x; Gij, Gkl, W_w)
+ *
+ * where C_{type}(x; Gij, Gkl, W) is exactly what baryon_general_two_point
+ * returns for site x with project_mn = W.
+ *
+ * Conventions
+ *   dims        {Lx, Ly, Lz, Lt} of the local lattice; volume = product.
+ *   site_order  0: lexicographic, site = ((t*Lz + z)*Ly + y)*Lx + x.
+ *               1: even-odd, site = parity*V/2 + ((t*Lz + z)*Ly + y)*Lx/2 + x/2
+ *                  with parity = (x+y+z+t) % 2 (PyQUDA propagators).
+ *   momenta     host int[nmom][3], integer components (any sign) in units
+ *               of 2pi/L_mu.
+ *   projectors  host double _Complex[nproj][4][4], row-major [m][n]; use
+ *               contract_gamma_matrix() for G(g).
+ *   out         device complex128 [nproj][nmom][Lt].
+ *   precision   CONTRACT_FP64: propagators are complex128, all arithmetic FP64.
+ *               CONTRACT_FP32: propagators are complex64; the per-site
+ *               contraction runs in FP32 on power-of-two rescaled blocks and
+ *               the timeslice/momentum sums stay FP64 (compensated).
+ *   workspace   device scratch of contract_project_workspace_bytes() bytes,
+ *               or NULL to let the library allocate it stream-ordered.
+ *   stream      a hipStream_t (NULL = null stream).
+ *
+ * Return value: 0 on success, nonzero on error; contract_last_error()
+ * describes the most recent error of the calling thread.  These entry points
+ * never exit the process.
+ */
+#pragma once
+
+#include <stddef.h>
+#ifndef __cplusplus
+#include <complex.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { CONTRACT_FP64 = 0, CONTRACT_FP32 = 1 };
+enum { CONTRACT_LEXICOGRAPHIC = 0, CONTRACT_EVEN_ODD = 1 };
+
+/* Message of the calling thread's most recent error ("" if none). */
+const char *contract_last_error(void);
+
+/* Dense DeGrand-Rossi G(g), g = 0..15, written as [4][4] row-major. */
+int contract_gamma_matrix(int g, double _Complex *out);
+
+/* Device scratch needed by the projected entry points for this lattice. */
+size_t contract_project_workspace_bytes(const int dims[4], int site_order, int nmom);
+
+/* Baryon two-point, summed over `nterm` contraction terms (e.g. the direct and
+ * exchange terms of a proton), for `nproj` insertions, projected to
+ * timeslices and momenta.  contract_types[t] in 0..5 (BaryonContractType). */
+int contract_baryon_twopoint_project(void *out, const void *propag_i, const void *propag_j, const void *propag_n,
+                                     int precision, const int *contract_types, const double _Complex *weights,
+                                     int nterm, int gamma_ij, int gamma_kl, const double _Complex *projectors,
+                                     int nproj, const int *momenta, int nmom, const int dims[4], int site_order,
+                                     void *workspace, size_t workspace_bytes, void *stream);
+
+/* Profiling hook: while set, every projected entry point called from this
+ * thread records `start` (a hipEvent_t) on its stream right before its main
+ * contraction kernel and `stop` right after it, so a caller can time that
+ * kernel alone.  Pass NULL, NULL to clear. */
+int contract_set_kernel_timer(void *start, void *stop);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,121 @@
+"""Host-side checks of libcontract.so that need no GPU: the library loads,
+exports every function include/*.h declares, and rejects bad arguments the
+way the reference does (print + exit(-1), or a silent no-op) before any HIP
+call is made."""
+
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_headers_declare_expected_abi():
+    assert declared_functions("contract.h") == sorted([
+        "init", "meson_two_point", "meson_all_source_two_point", "meson_all_sink_two_point", "baryon_diquark",
+        "baryon_two_point", "baryon_general_two_point", "baryon_sequential_two_point"])
+
+
+def test_library_exports_every_declared_symbol():
+    import contract_amd
+    L = contract_amd.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", contract_amd.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    for h in ("contract.h", "contract_ext.h"):
+        for name in declared_functions(h):
+            assert name in exported, (h, name)
+            getattr(L, name)
+    assert set(contract_amd.ABI_SYMBOLS) | set(contract_amd.EXT_SYMBOLS) <= exported
+
+
+def test_enums_match_reference_values():
+    from contract_amd.pycontract import BaryonContractType as T, BaryonSequentialType as S
+    assert [T.IK_JL_NM, T.IK_JM_NL, T.IL_JK_NM, T.IL_JM_NK, T.IM_JK_NL, T.IM_JL_NK] == list(range(6))
+    assert (T.IK_JL_MN, T.IK_JN_ML, T.IL_JK_MN, T.IL_JN_MK, T.IN_JK_ML, T.IN_JL_MK) == tuple(range(6))
+    assert (T.AD_BE_CF, T.AD_BF_CE, T.AE_BD_CF, T.AE_BF_CD, T.AF_BD_CE, T.AF_BE_CD) == tuple(range(6))
+    assert (S.SEQUENTIAL_I, S.SEQUENTIAL_J, S.SEQUENTIAL_N) == (0, 1, 2)
+
+
+def test_gamma_matrix_matches_oracle():
+    import numpy as np
+    import oracle
+    from contract_amd.pycontract import gamma_matrix
+    for g in range(16):
+        assert np.array_equal(gamma_matrix(g), oracle.gamma(g))
+
+
+def run_abi(snippet):
+    code = ("import ctypes, sys; sys.path.insert(0, %r)\n"
+            "import contract_amd; L = contract_amd.lib()\n" % ROOT) + snippet + "\nprint('RETURNED')"
+    return subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.parametrize("call,msg", [
+    ("L.meson_two_point(None, None, None, 6, 5, 5)", "multiple of TILES_PER_BLOCK"),
+    ("L.baryon_two_point(None, None, None, None, 6, 8, 5, 5, 8)", "Invalid contract_type"),
+    ("L.baryon_two_point(None, None, None, None, 0, 8, 5, 5, 16)", "Invalid gamma_mn"),
+    ("L.baryon_two_point(None, None, None, None, 0, 10, 5, 5, 8)", "multiple of TILES_PER_BLOCK"),
+    ("L.baryon_two_point(None, None, None, None, 0, 8, 17, 5, 8)", "Invalid gamma_ij"),
+    ("L.meson_two_point(None, None, None, 8, 5, -1)", "Invalid gamma_kl"),
+    ("L.baryon_sequential_two_point(None, None, None, 9, 0, 8, 5, 5, 8)", "Invalid contract_type"),
+    ("L.baryon_diquark(None, None, None, 7, 5, 5)", "multiple of TILES_PER_BLOCK"),
+])
+def test_invalid_arguments_exit(call, msg):
+    r = run_abi(call)
+    assert r.returncode == 255, (r.returncode, r.stdout, r.stderr)
+    assert msg in r.stderr
+    assert "RETURNED" not in r.stdout
+
+
+@pytest.mark.parametrize("call", [
+    "L.baryon_sequential_two_point(None, None, None, 0, 3, 8, 5, 5, 8)",   # lib/contract.cpp:60
+    "L.baryon_diquark(None, None, None, 8, 16, 5)",                        # lib/baryon/diquark.cu:93
+])
+def test_silent_noops(call):
+    r = run_abi(call)
+    assert r.returncode == 0 and "RETURNED" in r.stdout, (r.stdout, r.stderr)
+
+
+def test_extension_rejects_bad_arguments_with_codes():
+    import ctypes
+    import numpy as np
+    import contract_amd
+    L = contract_amd.lib()
+    vp = ctypes.c_void_p
+    dims = np.array([4, 4, 4, 8], np.int32)
+    mom = np.zeros((1, 3), np.int32)
+    types = np.array([0], np.int32)
+    w = np.array([1.0], np.complex128)
+    proj = np.eye(4, dtype=np.complex128)
+
+    def call(**kw):
+        a = dict(out=1, pi=1, pj=1, pn=1, prec=0, types=types.ctypes.data, w=w.ctypes.data, nterm=1, gij=5, gkl=5,
+                 proj=proj.ctypes.data, nproj=1, mom=mom.ctypes.data, nmom=1, dims=dims.ctypes.data, order=0)
+        a.update(kw)
+        return L.contract_baryon_twopoint_project(a["out"], a["pi"], a["pj"], a["pn"], a["prec"], a["types"],
+                                                  a["w"], a["nterm"], a["gij"], a["gkl"], a["proj"], a["nproj"],
+                                                  a["mom"], a["nmom"], a["dims"], a["order"], None, 0, None)
+
+    assert call(nmom=0) != 0 and "nmom" in contract_amd.last_error()
+    assert call(prec=3) != 0 and "precision" in contract_amd.last_error()
+    assert call(nterm=7) != 0
+    bad = np.array([9], np.int32)
+    assert call(types=bad.ctypes.data) != 0 and "BaryonContractType" in contract_amd.last_error()
+    assert call(gij=16) != 0
+    assert call(order=2) != 0 and "site_order" in contract_amd.last_error()
+    odd = np.array([3, 4, 4, 8], np.int32)
+    assert call(dims=odd.ctypes.data, order=1) != 0 and "even" in contract_amd.last_error()
+    assert call(out=0) != 0
+    assert L.contract_project_workspace_bytes(dims.ctypes.data, 0, 27) > 0
+    assert L.contract_project_workspace_bytes(dims.ctypes.data, 0, 0) == 0
