@@ -88,14 +88,10 @@ __device__ __forceinline__ C baryon_open(const C *__restrict__ si, const C *__re
         load_block(a, sj, j, m);
       else
         load_block(a, si, i, m);
-      r[m] = group_allreduce_bits<C, 12>(dot9(a, acc));
+      r[m] = dot9(a, acc);
     }
-    const int msel = q >> 2;
-    C out = r[0];
-    out = (msel == 1) ? r[1] : out;
-    out = (msel == 2) ? r[2] : out;
-    out = (msel == 3) ? r[3] : out;
-    return out;
+    // sum over i (the lanes q ^ 4, q ^ 8) and keep m = i in lane (i, n)
+    return reduce_scatter4_hi(r, q);
   }
 }
 
@@ -122,30 +118,25 @@ template <int T>
 __global__ __launch_bounds__(WAVE) void baryon_site_kernel(BaryonSiteArgs a)
 {
   using C = double2;
-  __shared__ C lds[3][SITES_PER_WAVE * SITE_LDS];
+  constexpr int SL = site_lds<C>();
+  __shared__ C lds[3 * SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
-  const size_t first = (size_t)blockIdx.x * SITES_PER_WAVE;
-  const size_t left = a.volume - first;
-  const int nsites = left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
-
-  stage_sites(lds[0], (const C *)a.propag_i, first, nsites, lane);
-  stage_sites(lds[1], (const C *)a.propag_j, first, nsites, lane);
-  stage_sites(lds[2], (const C *)a.propag_n, first, nsites, lane);
-  __syncthreads();
-
-  const C R = baryon_open<T, C>(lds[0] + g * SITE_LDS, lds[1] + g * SITE_LDS, lds[2] + g * SITE_LDS, q, a.gij,
-                                a.gklT);
   const C w = cmake<C>(a.w_re[q], a.w_im[q]);
-  const C c = group16_allreduce(cmul(w, R));
-  if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
+  const C *P[3] = {(const C *)a.propag_i, (const C *)a.propag_j, (const C *)a.propag_n};
+  persistent_tiles<3>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
+    const C R = baryon_open<T, C>(lds + g * SL, lds + (SITES_PER_WAVE + g) * SL,
+                                  lds + (2 * SITES_PER_WAVE + g) * SL, q, a.gij, a.gklT);
+    const C c = group16_allreduce(cmul(w, R));
+    if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
+  });
 }
 
 void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream)
 {
-  const size_t blocks = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
-  if (blocks == 0) return;
-  const dim3 grid((unsigned)blocks), block(WAVE);
+  const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
+  if (ntiles == 0) return;
+  const dim3 grid((unsigned)persistent_grid(ntiles, 5)), block(WAVE);
   switch (type) {
   case 0: hipLaunchKernelGGL(baryon_site_kernel<0>, grid, block, 0, stream, a); break;
   case 1: hipLaunchKernelGGL(baryon_site_kernel<1>, grid, block, 0, stream, a); break;
@@ -165,7 +156,8 @@ void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream)
 //   D[p][q] += sum_{s<4} phase[p][s] * R[s][q]
 // with v_mfma_f64_16x16x4f64: A = phases (lane: row p = lane&15, k = site
 // lane>>4), B = R (lane: k = site lane>>4, column q = lane&15) -- exactly the
-// registers the contraction leaves in each lane.  Complex products take four
+// registers the contraction leaves in each lane.  The result D[p][q] lands in
+// lane 16*(p % 4) + q, accumulator register p / 4.  Complex products take four
 // real MFMAs.  Partial sums go to partial[item][p][16]; project_finish sums
 // them per timeslice with Neumaier compensation in a fixed order.
 // ---------------------------------------------------------------------------
@@ -195,7 +187,10 @@ __device__ __forceinline__ void site_coords(size_t s, const int (&dims)[4], int 
   }
 }
 
-__device__ __forceinline__ double2 momentum_phase(const ProjectArgs &pa, int p, int x, int y, int z)
+// Phase exp(-2 pi i p.x / L) of momentum p at spatial site (x, y, z), from
+// per-direction tables held in LDS (tab = [Lx | Ly | Lz] entries).
+__device__ __forceinline__ double2 momentum_phase(const ProjectArgs &pa, const double2 *tab, int p, int x, int y,
+                                                  int z)
 {
   if (p >= pa.np) return make_double2(0.0, 0.0);
   int kx = (pa.mom[p][0] * x) % pa.dims[0];
@@ -204,26 +199,30 @@ __device__ __forceinline__ double2 momentum_phase(const ProjectArgs &pa, int p, 
   kx += kx < 0 ? pa.dims[0] : 0;
   ky += ky < 0 ? pa.dims[1] : 0;
   kz += kz < 0 ? pa.dims[2] : 0;
-  const double2 wx = pa.phase_tab[kx];
-  const double2 wy = pa.phase_tab[pa.dims[0] + ky];
-  const double2 wz = pa.phase_tab[pa.dims[0] + pa.dims[1] + kz];
-  return cmul(cmul(wx, wy), wz);
+  return cmul(cmul(tab[kx], tab[pa.dims[0] + ky]), tab[pa.dims[0] + pa.dims[1] + kz]);
 }
 
 template <int NT>
 __device__ __forceinline__ void project_accumulate(dbl4 (&acc_re)[NT], dbl4 (&acc_im)[NT], const ProjectArgs &pa,
-                                                   double2 R, size_t site, int q)
+                                                   const double2 *tab, double2 R, size_t site, int q)
 {
   int x, y, z, t;
   site_coords(site, pa.dims, pa.order, x, y, z, t);
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
-    const double2 ph = momentum_phase(pa, tt * 16 + q, x, y, z);
+    const double2 ph = momentum_phase(pa, tab, tt * 16 + q, x, y, z);
     acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.x, R.x, acc_re[tt], 0, 0, 0);
     acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ph.y, R.y, acc_re[tt], 0, 0, 0);
     acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.x, R.y, acc_im[tt], 0, 0, 0);
     acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.y, R.x, acc_im[tt], 0, 0, 0);
   }
+}
+
+// Copy the phase tables of the three spatial directions into LDS.
+__device__ __forceinline__ void load_phase_tab(double2 *tab, const ProjectArgs &pa, int lane)
+{
+  const int n = pa.dims[0] + pa.dims[1] + pa.dims[2];
+  for (int e = lane; e < n; e += WAVE) tab[e] = pa.phase_tab[e];
 }
 
 template <int NT>
@@ -236,7 +235,7 @@ __device__ __forceinline__ void project_store(const dbl4 (&acc_re)[NT], const db
   for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int p = tt * 16 + g * 4 + r;
+      const int p = tt * 16 + g + 4 * r; // v_mfma_f64_16x16x4f64: lane (g, q) register r holds D[g + 4r][q]
       if (p < pa.np) dst[p * 16 + q] = make_double2(acc_re[tt][r], acc_im[tt][r]);
     }
 }
@@ -278,25 +277,41 @@ template <int NT, typename Real>
 __global__ __launch_bounds__(WAVE) void baryon_project_kernel(BaryonProjectArgs a, ProjectArgs pa)
 {
   using C = cplx<Real>;
-  __shared__ C lds[3][SITES_PER_WAVE * SITE_LDS];
+  constexpr int SL = site_lds<C>();
+  __shared__ C lds[3][SITES_PER_WAVE * SL];
+  __shared__ double2 tab[MAX_PHASE_TAB];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
   const size_t item = blockIdx.x;
   size_t first, last;
   item_range(pa, item, first, last);
+  load_phase_tab(tab, pa, lane);
 
   dbl4 acc_re[NT], acc_im[NT];
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) acc_re[tt] = acc_im[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-  for (size_t s = first; s < last; s += SITES_PER_WAVE) {
+  const C *P[3] = {(const C *)a.propag_i, (const C *)a.propag_j, (const C *)a.propag_n};
+  auto nsites_of = [&](size_t s) {
     const size_t left = last - s;
-    const int nsites = left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
-    stage_sites(lds[0], (const C *)a.propag_i, s, nsites, lane);
-    stage_sites(lds[1], (const C *)a.propag_j, s, nsites, lane);
-    stage_sites(lds[2], (const C *)a.propag_n, s, nsites, lane);
+    return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
+  };
+  Tile<C> buf[3];
+  if (first < last)
+#pragma unroll
+    for (int p = 0; p < 3; ++p) buf[p].load(P[p], first, nsites_of(first), lane);
+
+  for (size_t s = first; s < last; s += SITES_PER_WAVE) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) buf[p].store(lds[p], lane);
     __syncthreads();
-    C *si = lds[0] + g * SITE_LDS, *sj = lds[1] + g * SITE_LDS, *sn = lds[2] + g * SITE_LDS;
+    const int nsites = nsites_of(s);
+    const size_t next = s + SITES_PER_WAVE;
+    if (next < last)
+#pragma unroll
+      for (int p = 0; p < 3; ++p) buf[p].load(P[p], next, nsites_of(next), lane);
+
+    C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[2] + g * SL;
     int ex = 0;
     if constexpr (sizeof(Real) == 4) {
       ex = normalise_site(si, q) + normalise_site(sj, q) + normalise_site(sn, q);
@@ -309,7 +324,7 @@ __global__ __launch_bounds__(WAVE) void baryon_project_kernel(BaryonProjectArgs 
     }
     if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
     const size_t site = (g < nsites) ? s + g : s; // padded lanes carry R = 0
-    project_accumulate<NT>(acc_re, acc_im, pa, R, site, q);
+    project_accumulate<NT>(acc_re, acc_im, pa, tab, R, site, q);
     __syncthreads();
   }
   project_store<NT>(acc_re, acc_im, pa, item, lane);

@@ -14,6 +14,11 @@ namespace contract_amd
 constexpr int MAX_TERMS = 6;
 constexpr int MAX_MOM = 64;
 constexpr int MAX_PROJ_PER_LAUNCH = 8;
+constexpr int MAX_PHASE_TAB = 3 * 128; // Lx + Ly + Lz of a projected lattice
+
+int device_cu_count();
+// Grid of a persistent staged-tile kernel: min(ntiles, CUs x waves_per_cu).
+size_t persistent_grid(size_t ntiles, int waves_per_cu);
 
 // ---- baryon, drop-in per-site form -------------------------------------
 struct BaryonSiteArgs {

@@ -148,43 +148,108 @@ template <typename C> __device__ __forceinline__ C group16_reduce_scatter(C (&v)
 }
 
 // ---------------------------------------------------------------------------
-// HBM -> LDS staging of SITES_PER_WAVE consecutive sites of one propagator.
-// Source layout is the ABI's site-major [V][144] complex.  The LDS copy keeps
-// the same order with a site stride of SITE_LDS complex entries (144 + pad).
-// Sites beyond `nsites` (ragged tail) are zero-filled so the arithmetic on
-// them is harmless; their results are never stored.
+// HBM -> registers -> LDS staging of one tile (SITES_PER_WAVE consecutive
+// sites) of one propagator.  The ABI layout is site-major [V][144] complex,
+// so a tile is one contiguous run of 4 x 144 complex; it moves as 16-byte
+// chunks, chunk e = it * 64 + lane, so every load instruction reads 1 KiB of
+// consecutive HBM.  Loading into registers first lets a wave keep the next
+// tile in flight while it computes on the current one from LDS.  In LDS each
+// site is padded by one 16-byte chunk so the four sites of a tile start on
+// different banks.  Sites past `nsites` (ragged tail) are zero-filled.
 // ---------------------------------------------------------------------------
-constexpr int SITE_LDS = SITE_ELEMS + 1; // pad one complex: successive sites start on different banks
+template <typename C> struct Tile {
+  static constexpr int SITE16 = SITE_ELEMS * (int)sizeof(C) / 16;     // 144 (FP64) / 72 (FP32)
+  static constexpr int LDS16 = SITE16 + 1;                             // padded site stride
+  static constexpr int LDS_ELEMS = LDS16 * 16 / (int)sizeof(C);       // per site, in complex
+  static constexpr int CHUNKS = SITES_PER_WAVE * SITE16;              // per tile
+  static constexpr int PER_LANE = (CHUNKS + WAVE - 1) / WAVE;         // 9 / 5
+  uint4 v[PER_LANE];
+
+  __device__ __forceinline__ void load(const C *__restrict__ src, size_t first_site, int nsites, int lane)
+  {
+    const uint4 *base = reinterpret_cast<const uint4 *>(src + first_site * SITE_ELEMS);
+    const int total = nsites * SITE16;
+#pragma unroll
+    for (int it = 0; it < PER_LANE; ++it) {
+      const int e = it * WAVE + lane;
+      v[it] = (e < total) ? base[e] : make_uint4(0, 0, 0, 0);
+    }
+  }
+
+  __device__ __forceinline__ void store(C *__restrict__ lds, int lane) const
+  {
+    uint4 *l = reinterpret_cast<uint4 *>(lds);
+#pragma unroll
+    for (int it = 0; it < PER_LANE; ++it) {
+      const int e = it * WAVE + lane;
+      if (CHUNKS % WAVE == 0 || e < CHUNKS) {
+        const int s = e / SITE16;
+        l[s * LDS16 + (e - s * SITE16)] = v[it];
+      }
+    }
+  }
+};
+
+template <typename C> constexpr int site_lds() { return Tile<C>::LDS_ELEMS; }
+constexpr int SITE_LDS = Tile<double2>::LDS_ELEMS; // FP64 padded site stride
 
 template <typename C>
 __device__ __forceinline__ void stage_sites(C *__restrict__ lds, const C *__restrict__ src, size_t first_site,
                                             int nsites, int lane)
 {
-  const C *base = src + first_site * SITE_ELEMS;
-  const int total = nsites * SITE_ELEMS;
-#pragma unroll
-  for (int it = 0; it < SITES_PER_WAVE * SITE_ELEMS / WAVE; ++it) {
-    const int e = it * WAVE + lane; // element within the 4-site chunk
-    const int s = e / SITE_ELEMS;
-    const int r = e - s * SITE_ELEMS;
-    C v = (e < total) ? base[e] : czero<C>();
-    lds[s * SITE_LDS + r] = v;
-  }
+  Tile<C> t;
+  t.load(src, first_site, nsites, lane);
+  t.store(lds, lane);
 }
 
-// LDS -> HBM: the inverse of stage_sites, for propagator-shaped outputs.
+// LDS -> HBM: the inverse of staging, for propagator-shaped outputs.
 template <typename C>
 __device__ __forceinline__ void unstage_sites(C *__restrict__ dst, const C *__restrict__ lds, size_t first_site,
                                               int nsites, int lane)
 {
-  C *base = dst + first_site * SITE_ELEMS;
-  const int total = nsites * SITE_ELEMS;
+  using T = Tile<C>;
+  uint4 *base = reinterpret_cast<uint4 *>(dst + first_site * SITE_ELEMS);
+  const uint4 *l = reinterpret_cast<const uint4 *>(lds);
+  const int total = nsites * T::SITE16;
 #pragma unroll
-  for (int it = 0; it < SITES_PER_WAVE * SITE_ELEMS / WAVE; ++it) {
+  for (int it = 0; it < T::PER_LANE; ++it) {
     const int e = it * WAVE + lane;
-    const int s = e / SITE_ELEMS;
-    const int r = e - s * SITE_ELEMS;
-    if (e < total) base[e] = lds[s * SITE_LDS + r];
+    if (e < total) {
+      const int s = e / T::SITE16;
+      base[e] = l[s * T::LDS16 + (e - s * T::SITE16)];
+    }
+  }
+}
+
+// Persistent grid-stride loop over the 4-site tiles of `volume` sites for a
+// kernel that stages NP propagators.  The loads of tile k+1 are issued into
+// registers before body(k) runs, so they overlap the contraction of tile k;
+// body(first_site, nsites) reads the staged sites from lds[p * 4 * SL ...].
+template <int NP, typename C, typename Body>
+__device__ __forceinline__ void persistent_tiles(C *__restrict__ lds, const C *const (&P)[NP], size_t volume,
+                                                 int lane, Body &&body)
+{
+  constexpr int SL = Tile<C>::LDS_ELEMS;
+  const size_t ntiles = (volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
+  auto nsites_of = [&](size_t t) {
+    const size_t left = volume - t * SITES_PER_WAVE;
+    return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
+  };
+  Tile<C> buf[NP];
+  size_t tile = blockIdx.x;
+  if (tile < ntiles)
+#pragma unroll
+    for (int p = 0; p < NP; ++p) buf[p].load(P[p], tile * SITES_PER_WAVE, nsites_of(tile), lane);
+  for (; tile < ntiles; tile += gridDim.x) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) buf[p].store(lds + p * SITES_PER_WAVE * SL, lane);
+    __syncthreads();
+    const size_t next = tile + gridDim.x;
+    if (next < ntiles)
+#pragma unroll
+      for (int p = 0; p < NP; ++p) buf[p].load(P[p], next * SITES_PER_WAVE, nsites_of(next), lane);
+    body(tile * SITES_PER_WAVE, nsites_of(tile));
+    __syncthreads();
   }
 }
 
@@ -241,13 +306,24 @@ struct SparseSpin {
 // Row lookups with a lane-dependent row go through selects, so the argument
 // block stays in scalar registers instead of being copied to scratch for
 // indexing.
-template <typename T> __device__ __forceinline__ T sel4(const T (&a)[4], int i)
+template <typename T> __device__ __forceinline__ T mux4(T a0, T a1, T a2, T a3, int i)
 {
-  T v = a[0];
-  v = (i == 1) ? a[1] : v;
-  v = (i == 2) ? a[2] : v;
-  v = (i == 3) ? a[3] : v;
-  return v;
+  // two levels of bit-selects; an equality chain gets turned back into a
+  // dynamically indexed stack array (scratch) by the optimiser
+  const T lo = (i & 1) ? a1 : a0;
+  const T hi = (i & 1) ? a3 : a2;
+  return (i & 2) ? hi : lo;
+}
+template <typename T> __device__ __forceinline__ T sel4(const T (&a)[4], int i) { return mux4(a[0], a[1], a[2], a[3], i); }
+
+// Reduce-scatter over the four lanes of a site group that differ in bits 2
+// and 3 of q (i = q >> 2): lane i returns sum over the four lanes of r[i].
+template <typename C> __device__ __forceinline__ C reduce_scatter4_hi(const C (&r)[4], int q)
+{
+  const bool h1 = q & 8, h0 = q & 4;
+  C a0 = cadd(h1 ? r[2] : r[0], shfl_xor_c(h1 ? r[0] : r[2], 8));
+  C a1 = cadd(h1 ? r[3] : r[1], shfl_xor_c(h1 ? r[1] : r[3], 8));
+  return cadd(h0 ? a1 : a0, shfl_xor_c(h0 ? a0 : a1, 4));
 }
 
 __device__ __forceinline__ int sparse_col(const SparseSpin &g, int row) { return sel4(g.col, row); }

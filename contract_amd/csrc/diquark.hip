@@ -14,44 +14,41 @@ namespace contract_amd
 __global__ __launch_bounds__(WAVE) void diquark_kernel(DiquarkArgs a)
 {
   using C = double2;
-  __shared__ C lds[2][SITES_PER_WAVE * SITE_LDS];
+  constexpr int SL = site_lds<C>();
+  __shared__ C lds[2 * SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
-  const size_t first = (size_t)blockIdx.x * SITES_PER_WAVE;
-  const size_t left = a.volume - first;
-  const int nsites = left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
-  stage_sites(lds[0], (const C *)a.propag_i, first, nsites, lane);
-  stage_sites(lds[1], (const C *)a.propag_j, first, nsites, lane);
-  __syncthreads();
-
-  const C *si = lds[0] + g * SITE_LDS, *sj = lds[1] + g * SITE_LDS;
   const int m = q >> 2, l = q & 3;
   const int k = sparse_col(a.gklT, l);
   const C gl = sparse_val<C>(a.gklT, l);
-  C acc[9], x[9], y[9];
-#pragma unroll
-  for (int e = 0; e < 9; ++e) acc[e] = czero<C>();
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int j = sparse_col(a.gij, i);
-    load_block(x, si, i, k);
-    load_block(y, sj, j, m);
-    cof_acc(acc, x, y, cmul(sparse_val<C>(a.gij, i), gl));
-  }
   const double s5 = ((m >> 1) ^ (l >> 1)) ? -1.0 : 1.0; // g5[m] g5[l]
-  __syncthreads();
-  C *row = lds[0] + g * SITE_LDS + q * 9;
+  const C *P[2] = {(const C *)a.propag_i, (const C *)a.propag_j};
+  persistent_tiles<2>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
+    const C *si = lds + g * SL, *sj = lds + (SITES_PER_WAVE + g) * SL;
+    C acc[9], x[9], y[9];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) row[e] = cmake<C>(s5 * acc[e].x, -s5 * acc[e].y);
-  __syncthreads();
-  unstage_sites(static_cast<C *>(a.diquark), lds[0], first, nsites, lane);
+    for (int e = 0; e < 9; ++e) acc[e] = czero<C>();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int j = sparse_col(a.gij, i);
+      load_block(x, si, i, k);
+      load_block(y, sj, j, m);
+      cof_acc(acc, x, y, cmul(sparse_val<C>(a.gij, i), gl));
+    }
+    __syncthreads();
+    C *row = lds + g * SL + q * 9;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) row[e] = cmake<C>(s5 * acc[e].x, -s5 * acc[e].y);
+    __syncthreads();
+    unstage_sites(static_cast<C *>(a.diquark), lds, first, nsites, lane);
+  });
 }
 
 void launch_diquark(const DiquarkArgs &a, hipStream_t stream)
 {
-  const size_t blocks = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
-  if (blocks == 0) return;
-  hipLaunchKernelGGL(diquark_kernel, dim3((unsigned)blocks), dim3(WAVE), 0, stream, a);
+  const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(diquark_kernel, dim3((unsigned)persistent_grid(ntiles, 8)), dim3(WAVE), 0, stream, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -94,90 +91,87 @@ template <int T, int X>
 __global__ __launch_bounds__(WAVE) void sequential_kernel(SequentialArgs a)
 {
   using C = double2;
-  __shared__ C lds[2][SITES_PER_WAVE * SITE_LDS];
+  constexpr int SL = site_lds<C>();
+  __shared__ C lds[2 * SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
-  const size_t first = (size_t)blockIdx.x * SITES_PER_WAVE;
-  const size_t left = a.volume - first;
-  const int nsites = left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
-  stage_sites(lds[0], (const C *)a.in0, first, nsites, lane);
-  stage_sites(lds[1], (const C *)a.in1, first, nsites, lane);
-  __syncthreads();
+  const C *P[2] = {(const C *)a.in0, (const C *)a.in1};
+  persistent_tiles<2>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
+    const C *s0 = lds + g * SL, *s1 = lds + (SITES_PER_WAVE + g) * SL;
+    const int r = q >> 2, c = q & 3;
+    constexpr int LX = seq_slot(T, X);
+    C acc[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) acc[e] = czero<C>();
 
-  const C *s0 = lds[0] + g * SITE_LDS, *s1 = lds[1] + g * SITE_LDS;
-  const int r = q >> 2, c = q & 3;
-  constexpr int LX = seq_slot(T, X);
-  C acc[9];
+    if constexpr (X == 2 && LX == SLM) {
+      // dC/dP_n[n,m] = Gmn[m,n] * sum_{il} Gij[i,j] Gkl[k,l] cof(P_i[i,X], P_j[j,Y])
+      const int i = q >> 2, l = q & 3;
+      const int j = sparse_col(a.gij, i), k = sparse_col(a.gklT, l);
+      const C w = cmul(sparse_val<C>(a.gij, i), sparse_val<C>(a.gklT, l));
+      seq_term<T, X>(acc, s0, s1, i, j, k, l, 0, 0, w);
 #pragma unroll
-  for (int e = 0; e < 9; ++e) acc[e] = czero<C>();
-
-  if constexpr (X == 2 && LX == SLM) {
-    // dC/dP_n[n,m] = Gmn[m,n] * sum_{il} Gij[i,j] Gkl[k,l] cof(P_i[i,X], P_j[j,Y])
-    const int i = q >> 2, l = q & 3;
-    const int j = sparse_col(a.gij, i), k = sparse_col(a.gklT, l);
-    const C w = cmul(sparse_val<C>(a.gij, i), sparse_val<C>(a.gklT, l));
-    seq_term<T, X>(acc, s0, s1, i, j, k, l, 0, 0, w);
+      for (int e = 0; e < 9; ++e) acc[e] = group16_allreduce(acc[e]);
+      const bool live = (sparse_col(a.gmnT, r) == c); // row [n = r][m = c]
+      const C gm = live ? sparse_val<C>(a.gmnT, r) : czero<C>();
 #pragma unroll
-    for (int e = 0; e < 9; ++e) acc[e] = group16_allreduce(acc[e]);
-    const bool live = (sparse_col(a.gmnT, r) == c); // row [n = r][m = c]
-    const C gm = live ? sparse_val<C>(a.gmnT, r) : czero<C>();
+      for (int e = 0; e < 9; ++e) acc[e] = cmul(gm, acc[e]);
+    } else {
 #pragma unroll
-    for (int e = 0; e < 9; ++e) acc[e] = cmul(gm, acc[e]);
-  } else {
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      int i = 0, j = 0, k = 0, l = 0, m = 0, n = 0;
-      C w;
-      // indices fixed by the output row
-      if constexpr (X == 0) {
-        i = r;
-        j = sparse_col(a.gij, i);
-        w = sparse_val<C>(a.gij, i);
-      } else if constexpr (X == 1) {
-        j = r;
-        i = sparse_col(a.gijT, j);
-        w = sparse_val<C>(a.gijT, j);
-      } else {
-        n = r;
-        m = sparse_col(a.gmnT, n);
-        w = sparse_val<C>(a.gmnT, n);
+      for (int f = 0; f < 4; ++f) {
+        int i = 0, j = 0, k = 0, l = 0, m = 0, n = 0;
+        C w;
+        // indices fixed by the output row
+        if constexpr (X == 0) {
+          i = r;
+          j = sparse_col(a.gij, i);
+          w = sparse_val<C>(a.gij, i);
+        } else if constexpr (X == 1) {
+          j = r;
+          i = sparse_col(a.gijT, j);
+          w = sparse_val<C>(a.gijT, j);
+        } else {
+          n = r;
+          m = sparse_col(a.gmnT, n);
+          w = sparse_val<C>(a.gmnT, n);
+        }
+        if constexpr (LX == SLK) {
+          k = c;
+          l = sparse_col(a.gkl, k);
+          w = cmul(w, sparse_val<C>(a.gkl, k));
+        } else if constexpr (LX == SLL) {
+          l = c;
+          k = sparse_col(a.gklT, l);
+          w = cmul(w, sparse_val<C>(a.gklT, l));
+        } else {
+          m = c;
+          n = sparse_col(a.gmn, m);
+          w = cmul(w, sparse_val<C>(a.gmn, m));
+        }
+        // the free index
+        if constexpr (X == 2) {
+          i = f;
+          j = sparse_col(a.gij, i);
+          w = cmul(w, sparse_val<C>(a.gij, i));
+        } else if constexpr (LX == SLM) {
+          l = f;
+          k = sparse_col(a.gklT, l);
+          w = cmul(w, sparse_val<C>(a.gklT, l));
+        } else {
+          n = f;
+          m = sparse_col(a.gmnT, n);
+          w = cmul(w, sparse_val<C>(a.gmnT, n));
+        }
+        seq_term<T, X>(acc, s0, s1, i, j, k, l, m, n, w);
       }
-      if constexpr (LX == SLK) {
-        k = c;
-        l = sparse_col(a.gkl, k);
-        w = cmul(w, sparse_val<C>(a.gkl, k));
-      } else if constexpr (LX == SLL) {
-        l = c;
-        k = sparse_col(a.gklT, l);
-        w = cmul(w, sparse_val<C>(a.gklT, l));
-      } else {
-        m = c;
-        n = sparse_col(a.gmn, m);
-        w = cmul(w, sparse_val<C>(a.gmn, m));
-      }
-      // the free index
-      if constexpr (X == 2) {
-        i = f;
-        j = sparse_col(a.gij, i);
-        w = cmul(w, sparse_val<C>(a.gij, i));
-      } else if constexpr (LX == SLM) {
-        l = f;
-        k = sparse_col(a.gklT, l);
-        w = cmul(w, sparse_val<C>(a.gklT, l));
-      } else {
-        n = f;
-        m = sparse_col(a.gmnT, n);
-        w = cmul(w, sparse_val<C>(a.gmnT, n));
-      }
-      seq_term<T, X>(acc, s0, s1, i, j, k, l, m, n, w);
     }
-  }
-  __syncthreads();
-  C *row = lds[0] + g * SITE_LDS + q * 9;
+    __syncthreads();
+    C *row = lds + g * SL + q * 9;
 #pragma unroll
-  for (int e = 0; e < 9; ++e) row[e] = acc[e];
-  __syncthreads();
-  unstage_sites(static_cast<C *>(a.out), lds[0], first, nsites, lane);
+    for (int e = 0; e < 9; ++e) row[e] = acc[e];
+    __syncthreads();
+    unstage_sites(static_cast<C *>(a.out), lds, first, nsites, lane);
+  });
 }
 
 template <int T>
@@ -192,9 +186,9 @@ static void launch_seq_t(int target, const SequentialArgs &a, dim3 grid, hipStre
 
 void launch_sequential(int type, int target, const SequentialArgs &a, hipStream_t stream)
 {
-  const size_t blocks = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
-  if (blocks == 0) return;
-  const dim3 grid((unsigned)blocks);
+  const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
+  if (ntiles == 0) return;
+  const dim3 grid((unsigned)persistent_grid(ntiles, 8));
   switch (type) {
   case 0: launch_seq_t<0>(target, a, grid, stream); break;
   case 1: launch_seq_t<1>(target, a, grid, stream); break;

@@ -40,6 +40,12 @@ int device_cu_count()
   return n;
 }
 
+size_t persistent_grid(size_t ntiles, int waves_per_cu)
+{
+  const size_t cap = (size_t)device_cu_count() * waves_per_cu;
+  return ntiles < cap ? ntiles : cap;
+}
+
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item)
@@ -187,6 +193,10 @@ int contract_baryon_twopoint_project(void *out, const void *propag_i, const void
   int rc;
   if ((rc = check_lattice(dims, site_order)) != CONTRACT_OK) return rc;
   if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
+  if (dims[0] + dims[1] + dims[2] > MAX_PHASE_TAB) {
+    set_last_error("Lx + Ly + Lz = %d exceeds %d", dims[0] + dims[1] + dims[2], MAX_PHASE_TAB);
+    return CONTRACT_ERR_ARG;
+  }
   if (!out || !propag_i || !propag_j || !propag_n) {
     set_last_error("NULL output or propagator pointer");
     return CONTRACT_ERR_ARG;

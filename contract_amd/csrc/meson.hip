@@ -16,34 +16,31 @@ namespace contract_amd
 __global__ __launch_bounds__(WAVE) void meson_kernel(MesonArgs a)
 {
   using C = double2;
-  __shared__ C lds[2][SITES_PER_WAVE * SITE_LDS];
+  constexpr int SL = site_lds<C>();
+  __shared__ C lds[2 * SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
-  const size_t first = (size_t)blockIdx.x * SITES_PER_WAVE;
-  const size_t left = a.volume - first;
-  const int nsites = left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
-  stage_sites(lds[0], (const C *)a.propag_i, first, nsites, lane);
-  stage_sites(lds[1], (const C *)a.propag_j, first, nsites, lane);
-  __syncthreads();
-
   const int i = q >> 2, l = q & 3;
   const int j = sparse_col(a.a, i), k = sparse_col(a.b, l);
-  C x[9], y[9];
-  load_block(x, lds[0] + g * SITE_LDS, i, k);
-  load_block(y, lds[1] + g * SITE_LDS, j, l);
-  C t = czero<C>();
-#pragma unroll
-  for (int e = 0; e < 9; ++e) t = cfma_conj(t, x[e], y[e]);
   const C w = cmul(sparse_val<C>(a.a, i), sparse_val<C>(a.b, l));
-  const C c = group16_allreduce(cmul(w, t));
-  if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
+  const C *P[2] = {(const C *)a.propag_i, (const C *)a.propag_j};
+  persistent_tiles<2>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
+    C x[9], y[9];
+    load_block(x, lds + g * SL, i, k);
+    load_block(y, lds + (SITES_PER_WAVE + g) * SL, j, l);
+    C t = czero<C>();
+#pragma unroll
+    for (int e = 0; e < 9; ++e) t = cfma_conj(t, x[e], y[e]);
+    const C c = group16_allreduce(cmul(w, t));
+    if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
+  });
 }
 
 void launch_meson(const MesonArgs &a, hipStream_t stream)
 {
-  const size_t blocks = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
-  if (blocks == 0) return;
-  hipLaunchKernelGGL(meson_kernel, dim3((unsigned)blocks), dim3(WAVE), 0, stream, a);
+  const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(meson_kernel, dim3((unsigned)persistent_grid(ntiles, 8)), dim3(WAVE), 0, stream, a);
 }
 
 // meson_all_source (Gij fixed, Gkl = G(g)) and meson_all_sink (Gkl fixed,
@@ -54,72 +51,63 @@ void launch_meson(const MesonArgs &a, hipStream_t stream)
 __global__ __launch_bounds__(WAVE) void meson_all_kernel(MesonAllArgs a)
 {
   using C = double2;
-  __shared__ C lds[2][SITES_PER_WAVE * SITE_LDS];
+  constexpr int SL = site_lds<C>();
+  __shared__ C lds[2 * SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
-  const size_t first = (size_t)blockIdx.x * SITES_PER_WAVE;
-  const size_t left = a.volume - first;
-  const int nsites = left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
-  stage_sites(lds[0], (const C *)a.propag_i, first, nsites, lane);
-  stage_sites(lds[1], (const C *)a.propag_j, first, nsites, lane);
-  __syncthreads();
-
-  const C *si = lds[0] + g * SITE_LDS, *sj = lds[1] + g * SITE_LDS;
-  const int i = q >> 2, l = q & 3;
-  C x[9], y[9], K[4], v[16];
-  if (!a.sink) {
-    // fixed row pair (i, j = a(i)); running k
-    const int j = sparse_col(a.fixed, i);
-    load_block(y, sj, j, l);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      load_block(x, si, i, k);
-      C t = czero<C>();
-#pragma unroll
-      for (int e = 0; e < 9; ++e) t = cfma_conj(t, x[e], y[e]);
-      K[k] = t;
-    }
-    const C fa = sparse_val<C>(a.fixed, i);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int k = sparse_col(a.run[r], l);
-      C sel = K[0];
-      sel = (k == 1) ? K[1] : sel;
-      sel = (k == 2) ? K[2] : sel;
-      sel = (k == 3) ? K[3] : sel;
-      v[r] = cmul(cmul(fa, sparse_val<C>(a.run[r], l)), sel);
-    }
-  } else {
-    const int k = sparse_col(a.fixed, l);
-    load_block(x, si, i, k);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
+  const C *P[2] = {(const C *)a.propag_i, (const C *)a.propag_j};
+  persistent_tiles<2>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
+    const C *si = lds + g * SL, *sj = lds + (SITES_PER_WAVE + g) * SL;
+    const int i = q >> 2, l = q & 3;
+    C x[9], y[9], K[4], v[16];
+    if (!a.sink) {
+      // fixed row pair (i, j = a(i)); running k
+      const int j = sparse_col(a.fixed, i);
       load_block(y, sj, j, l);
-      C t = czero<C>();
 #pragma unroll
-      for (int e = 0; e < 9; ++e) t = cfma_conj(t, x[e], y[e]);
-      K[j] = t;
-    }
-    const C fb = sparse_val<C>(a.fixed, l);
+      for (int k = 0; k < 4; ++k) {
+        load_block(x, si, i, k);
+        C t = czero<C>();
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int j = sparse_col(a.run[r], i);
-      C sel = K[0];
-      sel = (j == 1) ? K[1] : sel;
-      sel = (j == 2) ? K[2] : sel;
-      sel = (j == 3) ? K[3] : sel;
-      v[r] = cmul(cmul(sparse_val<C>(a.run[r], i), fb), sel);
+        for (int e = 0; e < 9; ++e) t = cfma_conj(t, x[e], y[e]);
+        K[k] = t;
+      }
+      const C fa = sparse_val<C>(a.fixed, i);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int k = sparse_col(a.run[r], l);
+        const C sel = mux4(K[0], K[1], K[2], K[3], k);
+        v[r] = cmul(cmul(fa, sparse_val<C>(a.run[r], l)), sel);
+      }
+    } else {
+      const int k = sparse_col(a.fixed, l);
+      load_block(x, si, i, k);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        load_block(y, sj, j, l);
+        C t = czero<C>();
+#pragma unroll
+        for (int e = 0; e < 9; ++e) t = cfma_conj(t, x[e], y[e]);
+        K[j] = t;
+      }
+      const C fb = sparse_val<C>(a.fixed, l);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int j = sparse_col(a.run[r], i);
+        const C sel = mux4(K[0], K[1], K[2], K[3], j);
+        v[r] = cmul(cmul(sparse_val<C>(a.run[r], i), fb), sel);
+      }
     }
-  }
-  const C c = group16_reduce_scatter(v, q);
-  if (g < nsites) static_cast<C *>(a.correl[q])[first + g] = c;
+    const C c = group16_reduce_scatter(v, q);
+    if (g < nsites) static_cast<C *>(a.correl[q])[first + g] = c;
+  });
 }
 
 void launch_meson_all(const MesonAllArgs &a, hipStream_t stream)
 {
   const size_t blocks = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
   if (blocks == 0) return;
-  hipLaunchKernelGGL(meson_all_kernel, dim3((unsigned)blocks), dim3(WAVE), 0, stream, a);
+  hipLaunchKernelGGL(meson_all_kernel, dim3((unsigned)persistent_grid(blocks, 8)), dim3(WAVE), 0, stream, a);
 }
 
 } // namespace contract_amd

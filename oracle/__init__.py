@@ -193,19 +193,21 @@ def project(site_corr, dims, momenta, order: int = 0) -> np.ndarray:
     return out
 
 
-def baryon_twopoint_project(propag_i, propag_j, propag_n, terms, gamma_ij: int, gamma_kl: int, gammas, dims,
+def baryon_twopoint_project(propag_i, propag_j, propag_n, terms, gamma_ij: int, gamma_kl: int, insertions, dims,
                             momenta, order: int = 0) -> np.ndarray:
-    """Fused workload: sum_terms w * baryon_two_point(type), for each gamma_mn in
-    ``gammas``, projected to [len(gammas), len(momenta), Lt]."""
+    """Fused workload: sum_terms w * baryon_general_two_point(type, W) for each
+    insertion W (a gamma index g -> G(g), or a 4x4 projector), projected to
+    [len(insertions), len(momenta), Lt]."""
     pi, pj, pn = _prop(propag_i), _prop(propag_j), _prop(propag_n)
     types = np.ascontiguousarray([t for t, _ in terms], dtype=np.int32)
     w = np.ascontiguousarray([complex(x) for _, x in terms], dtype=np.complex128)
-    g = np.ascontiguousarray(gammas, dtype=np.int32)
+    W = np.ascontiguousarray(np.stack([gamma(int(g)) if np.ndim(g) == 0 else np.asarray(g, np.complex128)
+                                       for g in insertions]))
     d = np.ascontiguousarray(dims, dtype=np.int32)
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
-    out = np.zeros((len(g), mom.shape[0], int(d[3])), np.complex128)
+    out = np.zeros((W.shape[0], mom.shape[0], int(d[3])), np.complex128)
     lib().oracle_baryon_twopoint_project(_ptr(out), _ptr(pi), _ptr(pj), _ptr(pn), _ptr(types), _ptr(w), len(types),
-                                         gamma_ij, gamma_kl, _ptr(g), len(g), _ptr(d), order, _ptr(mom),
+                                         gamma_ij, gamma_kl, _ptr(W), W.shape[0], _ptr(d), order, _ptr(mom),
                                          mom.shape[0])
     return out
 

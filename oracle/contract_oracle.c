@@ -482,29 +482,30 @@ void oracle_project(double *out, const double *site_corr, int nq, const int dims
   free(sum);
 }
 
-/* Fused baryon two-point workload (new API): for each insertion gamma_mn in
- * gammas[ng], the site correlator is sum_t weight[t] * baryon_two_point(type_t)
- * over the nterm contraction terms (e.g. direct + exchange of the proton),
- * then projected as in oracle_project.  out is [ng][np][Lt]. */
+/* Fused baryon two-point workload (new API): for each projector W_w (dense
+ * 4x4, row-major [m][n], W = G(g) for an insertion gamma), the site correlator
+ * is sum_t weight[t] * baryon_general_two_point(type_t, W_w), then projected as
+ * in oracle_project.  out is [nproj][np][Lt]. */
 void oracle_baryon_twopoint_project(double *out, const double *propag_i, const double *propag_j,
                                     const double *propag_n, const int *types, const double *weights, int nterm,
-                                    int gamma_ij, int gamma_kl, const int *gammas, int ng, const int dims[4], int order,
-                                    const int *mom, int np)
+                                    int gamma_ij, int gamma_kl, const double *projectors, int nproj, const int dims[4],
+                                    int order, const int *mom, int np)
 {
   const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
-  double *site = (double *)calloc(V * ng * 2, sizeof(double));
+  double *site = (double *)calloc(V * nproj * 2, sizeof(double));
   double *tmp = (double *)malloc(V * 2 * sizeof(double));
-  for (int g = 0; g < ng; ++g)
+  for (int w = 0; w < nproj; ++w)
     for (int t = 0; t < nterm; ++t) {
-      oracle_baryon_two_point(tmp, propag_i, propag_j, propag_n, types[t], V, gamma_ij, gamma_kl, gammas[g]);
+      oracle_baryon_general_two_point(tmp, propag_i, propag_j, propag_n, types[t], V, gamma_ij, gamma_kl,
+                                      projectors + (size_t)w * 32);
       const double wr = weights[2 * t], wi = weights[2 * t + 1];
       for (size_t x = 0; x < V; ++x) {
         const double a = tmp[2 * x], b = tmp[2 * x + 1];
-        site[(x * ng + g) * 2] += wr * a - wi * b;
-        site[(x * ng + g) * 2 + 1] += wr * b + wi * a;
+        site[(x * nproj + w) * 2] += wr * a - wi * b;
+        site[(x * nproj + w) * 2 + 1] += wr * b + wi * a;
       }
     }
-  oracle_project(out, site, ng, dims, order, mom, np);
+  oracle_project(out, site, nproj, dims, order, mom, np);
   free(tmp);
   free(site);
 }

@@ -1,8 +1,9 @@
 """GPU parity of the fused baryon two-point + timeslice/momentum projection
 (include/contract_ext.h) against the FP64 oracle.
 
-Per-timeslice relative error: for every insertion g and timeslice t,
-max_p |gpu - oracle| / max_p |oracle|.  Bars: 1e-12 for FP64 storage,
+Per-timeslice relative error: for every timeslice t, the largest deviation
+over all insertions and momenta divided by the largest magnitude over all
+insertions and momenta of that timeslice.  Bars: 1e-12 for FP64 storage,
 1e-6 for FP32 storage (FP32 contraction, FP64 accumulation), where the oracle
 runs in FP64 on the same FP32-stored values."""
 
@@ -19,8 +20,10 @@ PROTON = [(0, 1.0), (5, 1.0)]  # IK_JL_NM + IM_JL_NK on (u, d, u')
 
 
 def per_timeslice_error(got, want):
-    num = np.abs(got - want).max(axis=1)
-    den = np.abs(want).max(axis=1)
+    """max over t of  max_{insertion, momentum} |got - want| / max_{insertion, momentum} |want|
+    for arrays shaped [insertion][momentum][t]."""
+    num = np.abs(got - want).max(axis=(0, 1))
+    den = np.abs(want).max(axis=(0, 1))
     return (num / np.maximum(den, 1e-300)).max()
 
 
@@ -72,11 +75,12 @@ def test_ragged_slices(dev):
 
 
 def test_fp32_steep_decay(dev):
-    """FP32 storage with exp(-0.8|t|): 20 decades over 32 timeslices."""
-    got, want = run_case(dev, (4, 4, 4, 32), 0, PROTON, list(range(16)), [(0, 0, 0)], alpha=0.8, fp32=True)
+    """FP32 storage with exp(-0.8|t - t0|) over 64 timeslices: the three-propagator
+    product at the far end (~e^-150) is far below FP32's smallest normal."""
+    got, want = run_case(dev, (4, 4, 4, 64), 0, PROTON, list(range(16)), [(0, 0, 0)], alpha=0.8, fp32=True)
     err = per_timeslice_error(got, want)
     assert err < 1e-6, err
-    assert np.abs(want[:, 0, -1]).max() < 1e-60  # the far end really is tiny
+    assert np.abs(want[:, 0, -1]).max() < 1e-55
 
 
 def test_fp32_matches_fp64_closely(dev):
