@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="also run a plain 2 GiB device copy, the reference kernel for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the contraction has no multi-GPU path in this build "
@@ -188,6 +190,13 @@ def main():
     def step64():
         pc.baryonTwoPointProject(u, d, u2, TERMS, GAMMA_CG5, GAMMA_CG5, gammas, MOM27, DIMS, out=out64,
                                  workspace=ws)
+
+    if args.copy_ref:
+        src = torch.empty(2 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     wall64, kern64 = time_steps(torch, step64, args.steps, args.warmup, hip, timer)
     ref64 = out64.cpu().numpy()

@@ -114,6 +114,7 @@ static void baryon_site(void *correl, void *propag_i, void *propag_j, void *prop
   a.correl = correl;
   a.volume = volume;
   a.gij = sp(gamma_ij);
+  a.gkl = sp(gamma_kl);
   a.gklT = spT(gamma_kl);
   for (int q = 0; q < 16; ++q) {
     a.w_re[q] = W[q].real();

@@ -7,22 +7,21 @@
 //
 // where type I<X>_J<Y>_N<Z> names the source-spin index each propagator
 // carries (include/contract_enum.h:1-7; the SWAP_IJ / SWAP_KL forms of
-// include/baryon.cuh:41-52,95-105 reduce to this table).  Instead of
-// evaluating C for one Gmn at a time, a site group computes the open-spin
-// correlator R[n][m] = dC/dGmn[m,n] (16 numbers), from which any insertion or
-// projector follows as C = sum_mn W[m,n] R[n,m]:
+// include/baryon.cuh:41-52,95-105 reduce to this table).  All six types
+// factor through one intermediate, the spin-open diquark of P_i and P_j
 //
-//   Z = m (IK_JL_NM, IL_JK_NM):  X1[a,d] = sum_{il} g cof(P_i[i,X], P_j[j,Y])[a,d]
-//                                R[n][m] = sum_ad P_n[n,m](a,d) X1[a,d]
-//   Z in {k,l} (the other four):  W_in[a,d] = sum_l g cof(A[., a-slot], P_n[n, b-slot])
-//                                R[n][m] = sum_i sum_ad M[., m](a,d) W_in[a,d]
+//   D[x][y](a,d) = sum_i Gij[i,j(i)] cof(P_i[i,x], P_j[j(i),y])[a,d]
 //
-// with M the propagator that carries m and A the other one of (P_i, P_j).
-// That costs ~45 (type 1) or ~180 (type 2) complex multiply-adds per lane for
-// all sixteen insertions, against ~100-200 per lane per single insertion in
-// the reference's lane loop (include/baryon.cuh:55-82).
+// (cof = the colour cofactor with one epsilon slot open), after which each
+// type is a cheap contraction of D with P_n and Gkl into the open-spin
+// correlator R[n][m] = dC/dGmn[m,n]; any insertion or projector W then gives
+// C = sum_mn W[m,n] R[n,m].  D costs 64 cofactors per site and is shared by
+// every contraction term and every insertion; the reference's lane loop
+// (include/baryon.cuh:55-82) redoes its 36-144 cofactors per site for each
+// type and each insertion.
 //
-// Lane q of a 16-lane site group holds R[n][m] for q = m*4 + n.
+// Lane q of a 16-lane site group holds D[x][y] for q = 4x + y, and R[n][m]
+// for q = 4m + n.
 
 #include "baryon.hpp"
 #include "device.hpp"
@@ -40,72 +39,96 @@ __host__ __device__ constexpr int type_slot(int type, int prop)
   return tab[type][prop];
 }
 
-template <int T, typename C>
-__device__ __forceinline__ C baryon_open(const C *__restrict__ si, const C *__restrict__ sj, const C *__restrict__ sn,
-                                         int q, const SparseSpin &gij, const SparseSpin &gklT)
+// D[x][y] for lane q = 4x + y.  The i loop is uniform across the wave, so the
+// gamma row lookups are scalar.
+template <typename C>
+__device__ __forceinline__ void diquark_tensor(C (&D)[9], const C *__restrict__ si, const C *__restrict__ sj, int q,
+                                               const SparseSpin &gij)
 {
-  constexpr int X = type_slot(T, 0), Y = type_slot(T, 1), Z = type_slot(T, 2);
+  const int x = q >> 2, y = q & 3;
   C a[9], b[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) D[e] = czero<C>();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    load_block(a, si, i, x);
+    load_block(b, sj, gij.col[i], y);
+    cof_acc(D, a, b, cmake<C>((decltype(C::x))gij.re[i], (decltype(C::x))gij.im[i]));
+  }
+}
+
+// Per-lane constants of the Gkl tails, computed once per kernel.  For lane
+// (x, y): kx = col(x), ky = col(y) of Gkl (its pattern is symmetric), and the
+// values Gkl[x][kx] (u), Gkl[kx][x] (v), likewise for y.
+template <typename C> struct KlLane {
+  int kx, ky;
+  C ux, vx, uy, vy;
+  C w0, w2; // weights of D in the Z = m tails (IK_JL_NM, IL_JK_NM); 0 off-pattern
+  __device__ __forceinline__ KlLane(const SparseSpin &gkl, const SparseSpin &gklT, int q)
+  {
+    const int x = q >> 2, y = q & 3;
+    kx = sparse_col(gkl, x);
+    ky = sparse_col(gkl, y);
+    ux = sparse_val<C>(gkl, x);
+    vx = sparse_val<C>(gklT, x);
+    uy = sparse_val<C>(gkl, y);
+    vy = sparse_val<C>(gklT, y);
+    w0 = (x == ky) ? vy : czero<C>(); // D[k(l)][l] with l = y
+    w2 = (y == kx) ? vx : czero<C>(); // D[l][k(l)] with l = x
+  }
+};
+
+// Open-spin correlator of type T from D, returned in the R layout q = 4m + n.
+template <int T, typename C>
+__device__ __forceinline__ C baryon_tail(const C (&D)[9], const C *__restrict__ sn, int lane, const KlLane<C> &kl)
+{
+  const int q = lane & 15;
+  constexpr int Z = type_slot(T, 2);
+  C a[9];
   if constexpr (Z == LM) {
-    const int i = q >> 2, l = q & 3;
-    const int j = sparse_col(gij, i), k = sparse_col(gklT, l);
-    const C w = cmul(sparse_val<C>(gij, i), sparse_val<C>(gklT, l));
-    load_block(a, si, i, X == LK ? k : l);
-    load_block(b, sj, j, Y == LK ? k : l);
+    // R[n][m] = sum_ad P_n[n][m] X1, X1 = sum_l Gkl[k,l] D[k][l] (T = 0) or D[l][k] (T = 2)
+    const C w = (T == 0) ? kl.w0 : kl.w2;
     C x1[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) x1[e] = czero<C>();
-    cof_acc(x1, a, b, w);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) x1[e] = group16_allreduce(x1[e]);
-    const int m = q >> 2, n = q & 3;
-    load_block(a, sn, n, m);
+    for (int e = 0; e < 9; ++e) x1[e] = group16_allreduce(cmul(w, D[e]));
+    load_block(a, sn, q & 3, q >> 2);
     return dot9(a, x1);
   } else {
-    constexpr bool carrier_j = (Y == LM);     // which of P_i / P_j carries m
-    constexpr int LA = carrier_j ? X : Y;     // source letter of the other one
-    const int i = q >> 2, n = q & 3;
-    const int j = sparse_col(gij, i);
-    const C gi = sparse_val<C>(gij, i);
-    C acc[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) acc[e] = czero<C>();
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int k = sparse_col(gklT, l);
-      const C w = cmul(gi, sparse_val<C>(gklT, l));
-      if constexpr (carrier_j)
-        load_block(a, si, i, LA == LK ? k : l);
-      else
-        load_block(a, sj, j, LA == LK ? k : l);
-      load_block(b, sn, n, Z == LK ? k : l);
-      cof_acc(acc, a, b, w);
-    }
+    // lane (x, y) owns one (l, m) term of  R[n][m] = sum_l w_l sum_ad P_n[n][c] D[.][.]
+    //   T=1 IK_JM_NL: D[k][m], k = x, l = kx, c = l,  w = u_x   -> reduce over x
+    //   T=3 IL_JM_NK: D[l][m], l = x, c = kx,         w = v_x   -> reduce over x
+    //   T=4 IM_JK_NL: D[m][k], k = y, l = ky, c = l,  w = u_y   -> reduce over y
+    //   T=5 IM_JL_NK: D[m][l], l = y, c = ky,         w = v_y   -> reduce over y
+    constexpr bool over_x = (T == 1 || T == 3);
+    const int c = over_x ? kl.kx : kl.ky;
+    const C w = (T == 1) ? kl.ux : (T == 3) ? kl.vx : (T == 4) ? kl.uy : kl.vy;
     C r[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      if constexpr (carrier_j)
-        load_block(a, sj, j, m);
-      else
-        load_block(a, si, i, m);
-      r[m] = dot9(a, acc);
+    for (int n = 0; n < 4; ++n) {
+      load_block(a, sn, n, c);
+      r[n] = cmul(w, dot9(a, D));
     }
-    // sum over i (the lanes q ^ 4, q ^ 8) and keep m = i in lane (i, n)
-    return reduce_scatter4_hi(r, q);
+    if constexpr (over_x) {
+      // lane (x, y) ends with R[n = x][m = y]; move it to lane 4m + n
+      const C t = reduce_scatter4_hi(r, q);
+      const int src = (lane & ~15) | ((q & 3) << 2) | (q >> 2);
+      return cmake<C>(__shfl(t.x, src, 64), __shfl(t.y, src, 64));
+    } else {
+      return reduce_scatter4_lo(r, q); // lane (x, y) = (m, n)
+    }
   }
 }
 
 template <typename C>
-__device__ __forceinline__ C baryon_open_rt(int type, const C *si, const C *sj, const C *sn, int q,
-                                            const SparseSpin &gij, const SparseSpin &gklT)
+__device__ __forceinline__ C baryon_tail_rt(int type, const C (&D)[9], const C *sn, int lane, const KlLane<C> &kl)
 {
   switch (type) {
-  case 0: return baryon_open<0, C>(si, sj, sn, q, gij, gklT);
-  case 1: return baryon_open<1, C>(si, sj, sn, q, gij, gklT);
-  case 2: return baryon_open<2, C>(si, sj, sn, q, gij, gklT);
-  case 3: return baryon_open<3, C>(si, sj, sn, q, gij, gklT);
-  case 4: return baryon_open<4, C>(si, sj, sn, q, gij, gklT);
-  default: return baryon_open<5, C>(si, sj, sn, q, gij, gklT);
+  case 0: return baryon_tail<0, C>(D, sn, lane, kl);
+  case 1: return baryon_tail<1, C>(D, sn, lane, kl);
+  case 2: return baryon_tail<2, C>(D, sn, lane, kl);
+  case 3: return baryon_tail<3, C>(D, sn, lane, kl);
+  case 4: return baryon_tail<4, C>(D, sn, lane, kl);
+  default: return baryon_tail<5, C>(D, sn, lane, kl);
   }
 }
 
@@ -123,10 +146,12 @@ __global__ __launch_bounds__(WAVE) void baryon_site_kernel(BaryonSiteArgs a)
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
   const C w = cmake<C>(a.w_re[q], a.w_im[q]);
+  const KlLane<C> kl(a.gkl, a.gklT, q);
   const C *P[3] = {(const C *)a.propag_i, (const C *)a.propag_j, (const C *)a.propag_n};
   persistent_tiles<3>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
-    const C R = baryon_open<T, C>(lds + g * SL, lds + (SITES_PER_WAVE + g) * SL,
-                                  lds + (2 * SITES_PER_WAVE + g) * SL, q, a.gij, a.gklT);
+    C D[9];
+    diquark_tensor(D, lds + g * SL, lds + (SITES_PER_WAVE + g) * SL, q, a.gij);
+    const C R = baryon_tail<T, C>(D, lds + (2 * SITES_PER_WAVE + g) * SL, lane, kl);
     const C c = group16_allreduce(cmul(w, R));
     if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
   });
@@ -286,6 +311,7 @@ __global__ __launch_bounds__(WAVE) void baryon_project_kernel(BaryonProjectArgs 
   size_t first, last;
   item_range(pa, item, first, last);
   load_phase_tab(tab, pa, lane);
+  const KlLane<C> kl(a.gkl, a.gklT, q);
 
   dbl4 acc_re[NT], acc_im[NT];
 #pragma unroll
@@ -317,9 +343,11 @@ __global__ __launch_bounds__(WAVE) void baryon_project_kernel(BaryonProjectArgs 
       ex = normalise_site(si, q) + normalise_site(sj, q) + normalise_site(sn, q);
       __syncthreads();
     }
+    C D[9];
+    diquark_tensor(D, si, sj, q, a.gij);
     double2 R = make_double2(0.0, 0.0);
     for (int t = 0; t < a.nterm; ++t) {
-      const C r = baryon_open_rt<C>(a.types[t], si, sj, sn, q, a.gij, a.gklT);
+      const C r = baryon_tail_rt<C>(a.types[t], D, sn, lane, kl);
       R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)r.x, (double)r.y));
     }
     if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));

@@ -26,6 +26,7 @@ struct BaryonSiteArgs {
   void *correl;
   size_t volume;
   SparseSpin gij;  // Gij: row i -> (j, Gij[i,j])
+  SparseSpin gkl;  // Gkl: row k -> (l, Gkl[k,l])
   SparseSpin gklT; // Gkl^T: row l -> (k, Gkl[k,l])
   double w_re[16], w_im[16]; // projector W[m][n] at q = m*4 + n
 };
@@ -46,7 +47,7 @@ struct ProjectArgs {
 
 struct BaryonProjectArgs {
   const void *propag_i, *propag_j, *propag_n;
-  SparseSpin gij, gklT;
+  SparseSpin gij, gkl, gklT;
   int nterm;
   int types[MAX_TERMS];
   double w_re[MAX_TERMS], w_im[MAX_TERMS];

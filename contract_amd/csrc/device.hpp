@@ -326,6 +326,16 @@ template <typename C> __device__ __forceinline__ C reduce_scatter4_hi(const C (&
   return cadd(h0 ? a1 : a0, shfl_xor_c(h0 ? a0 : a1, 4));
 }
 
+// Same over the lanes that differ in bits 0 and 1 (y = q & 3): lane y returns
+// sum over the four lanes of r[y].
+template <typename C> __device__ __forceinline__ C reduce_scatter4_lo(const C (&r)[4], int q)
+{
+  const bool h1 = q & 2, h0 = q & 1;
+  C a0 = cadd(h1 ? r[2] : r[0], shfl_xor_c(h1 ? r[0] : r[2], 2));
+  C a1 = cadd(h1 ? r[3] : r[1], shfl_xor_c(h1 ? r[1] : r[3], 2));
+  return cadd(h0 ? a1 : a0, shfl_xor_c(h0 ? a0 : a1, 1));
+}
+
 __device__ __forceinline__ int sparse_col(const SparseSpin &g, int row) { return sel4(g.col, row); }
 
 template <typename C> __device__ __forceinline__ C sparse_val(const SparseSpin &g, int row)
