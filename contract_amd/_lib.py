@@ -12,7 +12,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libcontract.so")
+LIB_PATH = os.environ.get("CONTRACT_AMD_LIB") or os.path.join(HERE, "lib", "libcontract.so")
 
 _lib = None
 

@@ -18,7 +18,7 @@ void check_launch() { CONTRACT_HIP_CHECK(hipGetLastError()); }
 
 void check_contract_type(int t)
 {
-  // lib/baryon/baryon.cu:58-64: exit on an unknown type
+  // lib/baryon/baryon.cu:61-64: exit on an unknown type
   if (t < 0 || t > 5) {
     std::fprintf(stderr, "Error: Invalid contract_type value %d\n", t);
     std::exit(-1);
@@ -27,7 +27,7 @@ void check_contract_type(int t)
 
 void check_gamma_mn(int g)
 {
-  // lib/baryon/baryon.cu:40-46
+  // lib/baryon/baryon.cu:43-46
   if (g < 0 || g > 15) {
     std::fprintf(stderr, "Error: Invalid gamma_mn value %d\n", g);
     std::exit(-1);
@@ -90,7 +90,7 @@ void meson_all_sink_two_point(void **correl, void *propag_i, void *propag_j, uns
 
 void baryon_diquark(void *diquark, void *propag_i, void *propag_j, size_t volume, int gamma_ij, int gamma_kl)
 {
-  if (gamma_ij < 0 || gamma_ij > 15) return; // lib/baryon/diquark.cu:93: silent no-op
+  if (gamma_ij < 0 || gamma_ij > 15) return; // lib/baryon/diquark.cu:40: silent no-op
   abi_check_volume(volume);
   abi_check_gamma(gamma_kl, "gamma_kl");
   DiquarkArgs a;

@@ -46,14 +46,23 @@ __device__ __forceinline__ void diquark_tensor(C (&D)[9], const C *__restrict__ 
                                                const SparseSpin &gij)
 {
   const int x = q >> 2, y = q & 3;
-  C a[9], b[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) D[e] = czero<C>();
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
+    const C w = cmake<C>((decltype(C::x))gij.re[i], (decltype(C::x))gij.im[i]);
+#if CONTRACT_COF_STREAM
+    C b[9];
+    load_block(b, sj, gij.col[i], y);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) b[e] = cmul(w, b[e]);
+    cof_acc_stream(D, si + (i * 4 + x) * 9, b);
+#else
+    C a[9], b[9];
     load_block(a, si, i, x);
     load_block(b, sj, gij.col[i], y);
-    cof_acc(D, a, b, cmake<C>((decltype(C::x))gij.re[i], (decltype(C::x))gij.im[i]));
+    cof_acc(D, a, b, w);
+#endif
   }
 }
 
@@ -298,13 +307,22 @@ __device__ __forceinline__ int normalise_site(float2 *site_lds, int q)
   return ex;
 }
 
+#ifndef CONTRACT_WPE64
+#define CONTRACT_WPE64 1
+#endif
+#ifndef CONTRACT_WPE32
+#define CONTRACT_WPE32 2
+#endif
+
 template <int NT, typename Real>
-__global__ __launch_bounds__(WAVE) void baryon_project_kernel(BaryonProjectArgs a, ProjectArgs pa)
+__global__ __launch_bounds__(WAVE)
+__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT_WPE32))) void baryon_project_kernel(
+  BaryonProjectArgs a, ProjectArgs pa)
 {
   using C = cplx<Real>;
   constexpr int SL = site_lds<C>();
   __shared__ C lds[3][SITES_PER_WAVE * SL];
-  __shared__ double2 tab[MAX_PHASE_TAB];
+  extern __shared__ double2 tab[]; // Lx + Ly + Lz phase entries, sized at launch
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
   const size_t item = blockIdx.x;
@@ -363,11 +381,12 @@ static void launch_project_t(const BaryonProjectArgs &a, const ProjectArgs &pa, 
 {
   const int nt = (pa.np + 15) / 16;
   const dim3 grid((unsigned)items), block(WAVE);
+  const size_t tab_bytes = sizeof(double2) * (pa.dims[0] + pa.dims[1] + pa.dims[2]);
   switch (nt) {
-  case 1: hipLaunchKernelGGL((baryon_project_kernel<1, Real>), grid, block, 0, stream, a, pa); break;
-  case 2: hipLaunchKernelGGL((baryon_project_kernel<2, Real>), grid, block, 0, stream, a, pa); break;
-  case 3: hipLaunchKernelGGL((baryon_project_kernel<3, Real>), grid, block, 0, stream, a, pa); break;
-  default: hipLaunchKernelGGL((baryon_project_kernel<4, Real>), grid, block, 0, stream, a, pa); break;
+  case 1: hipLaunchKernelGGL((baryon_project_kernel<1, Real>), grid, block, tab_bytes, stream, a, pa); break;
+  case 2: hipLaunchKernelGGL((baryon_project_kernel<2, Real>), grid, block, tab_bytes, stream, a, pa); break;
+  case 3: hipLaunchKernelGGL((baryon_project_kernel<3, Real>), grid, block, tab_bytes, stream, a, pa); break;
+  default: hipLaunchKernelGGL((baryon_project_kernel<4, Real>), grid, block, tab_bytes, stream, a, pa); break;
   }
 }
 

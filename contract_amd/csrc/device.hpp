@@ -15,6 +15,10 @@
 namespace contract_amd
 {
 
+#ifndef CONTRACT_COF_STREAM
+#define CONTRACT_COF_STREAM 1
+#endif
+
 constexpr int NS = 4;
 constexpr int NC = 3;
 constexpr int SITE_ELEMS = 144; // [4][4][3][3] complex entries per site
@@ -280,6 +284,30 @@ template <typename C> __device__ __forceinline__ void cof_acc(C (&acc)[9], const
       t = cfms(t, Y[b * 3 + f], Z[c * 3 + e]);
       t = cfms(t, Y[c * 3 + e], Z[b * 3 + f]);
       acc[a * 3 + d] = cfma(acc[a * 3 + d], w, t);
+    }
+  }
+}
+
+// acc += cof(Y, Z) with Y read one entry at a time from LDS and Z held in
+// registers: every Y[b][e] feeds the four entries (a, d) with a != b, d != e,
+//   acc[a1][d1] += Y Z[a2][d2]   acc[a1][d2] -= Y Z[a2][d1]
+//   acc[a2][d1] -= Y Z[a1][d2]   acc[a2][d2] += Y Z[a1][d1]
+// with a1 = b+1, a2 = b+2, d1 = e+1, d2 = e+2 (mod 3).  Same 36 complex
+// products as cof_acc with one block fewer live in registers.
+template <typename C>
+__device__ __forceinline__ void cof_acc_stream(C (&acc)[9], const C *__restrict__ Yp, const C (&Z)[9])
+{
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const int a1 = (b + 1) % 3, a2 = (b + 2) % 3;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      const int d1 = (e + 1) % 3, d2 = (e + 2) % 3;
+      const C y = Yp[b * 3 + e];
+      acc[a1 * 3 + d1] = cfma(acc[a1 * 3 + d1], y, Z[a2 * 3 + d2]);
+      acc[a1 * 3 + d2] = cfms(acc[a1 * 3 + d2], y, Z[a2 * 3 + d1]);
+      acc[a2 * 3 + d1] = cfms(acc[a2 * 3 + d1], y, Z[a1 * 3 + d2]);
+      acc[a2 * 3 + d2] = cfma(acc[a2 * 3 + d2], y, Z[a1 * 3 + d1]);
     }
   }
 }

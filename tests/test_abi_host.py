@@ -80,7 +80,7 @@ def test_invalid_arguments_exit(call, msg):
 
 @pytest.mark.parametrize("call", [
     "L.baryon_sequential_two_point(None, None, None, 0, 3, 8, 5, 5, 8)",   # lib/contract.cpp:60
-    "L.baryon_diquark(None, None, None, 8, 16, 5)",                        # lib/baryon/diquark.cu:93
+    "L.baryon_diquark(None, None, None, 8, 16, 5)",                        # lib/baryon/diquark.cu:40
 ])
 def test_silent_noops(call):
     r = run_abi(call)
