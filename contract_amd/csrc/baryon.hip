@@ -88,18 +88,41 @@ template <typename C> struct KlLane {
 };
 
 // Open-spin correlator of type T from D, returned in the R layout q = 4m + n.
-template <int T, typename C>
-__device__ __forceinline__ C baryon_tail(const C (&D)[9], const C *__restrict__ sn, int lane, const KlLane<C> &kl)
+// `scr` is 45 complex of LDS owned by this 16-lane group (the group's
+// consumed P_i / P_j staging area), used by the Z = m types to sum D over
+// four lanes without a 9-entry lane all-reduce.
+// LDS_SUM = false sums with a 16-lane all-reduce instead (fewer registers
+// in the single-type drop-in kernels, where the LDS round trips cost more).
+template <int T, typename C, bool LDS_SUM = true>
+__device__ __forceinline__ C baryon_tail(const C (&D)[9], const C *__restrict__ sn, C *__restrict__ scr, int lane,
+                                         const KlLane<C> &kl)
 {
   const int q = lane & 15;
   constexpr int Z = type_slot(T, 2);
   C a[9];
-  if constexpr (Z == LM) {
-    // R[n][m] = sum_ad P_n[n][m] X1, X1 = sum_l Gkl[k,l] D[k][l] (T = 0) or D[l][k] (T = 2)
+  if constexpr (Z == LM && !LDS_SUM) {
     const C w = (T == 0) ? kl.w0 : kl.w2;
     C x1[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) x1[e] = group16_allreduce(cmul(w, D[e]));
+    load_block(a, sn, q & 3, q >> 2);
+    return dot9(a, x1);
+  } else if constexpr (Z == LM) {
+    // R[n][m] = sum_ad P_n[n][m] X1, X1 = sum_l Gkl[k,l] D[k][l] (T = 0) or D[l][k] (T = 2):
+    // the four lanes holding a D[k(l)][l] (or D[l][k(l)]) post it weighted to
+    // LDS slot l, nine lanes sum the slots entry-wise, every lane reads X1.
+    const C w = (T == 0) ? kl.w0 : kl.w2;
+    const int slot = (T == 0) ? (q & 3) : (q >> 2);
+    __syncthreads();
+    if (w.x != 0 || w.y != 0)
+#pragma unroll
+      for (int e = 0; e < 9; ++e) scr[slot * 9 + e] = cmul(w, D[e]);
+    __syncthreads();
+    if (q < 9) scr[36 + q] = cadd(cadd(scr[q], scr[9 + q]), cadd(scr[18 + q], scr[27 + q]));
+    __syncthreads();
+    C x1[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) x1[e] = scr[36 + e];
     load_block(a, sn, q & 3, q >> 2);
     return dot9(a, x1);
   } else {
@@ -129,15 +152,16 @@ __device__ __forceinline__ C baryon_tail(const C (&D)[9], const C *__restrict__ 
 }
 
 template <typename C>
-__device__ __forceinline__ C baryon_tail_rt(int type, const C (&D)[9], const C *sn, int lane, const KlLane<C> &kl)
+__device__ __forceinline__ C baryon_tail_rt(int type, const C (&D)[9], const C *sn, C *scr, int lane,
+                                            const KlLane<C> &kl)
 {
   switch (type) {
-  case 0: return baryon_tail<0, C>(D, sn, lane, kl);
-  case 1: return baryon_tail<1, C>(D, sn, lane, kl);
-  case 2: return baryon_tail<2, C>(D, sn, lane, kl);
-  case 3: return baryon_tail<3, C>(D, sn, lane, kl);
-  case 4: return baryon_tail<4, C>(D, sn, lane, kl);
-  default: return baryon_tail<5, C>(D, sn, lane, kl);
+  case 0: return baryon_tail<0, C>(D, sn, scr, lane, kl);
+  case 1: return baryon_tail<1, C>(D, sn, scr, lane, kl);
+  case 2: return baryon_tail<2, C>(D, sn, scr, lane, kl);
+  case 3: return baryon_tail<3, C>(D, sn, scr, lane, kl);
+  case 4: return baryon_tail<4, C>(D, sn, scr, lane, kl);
+  default: return baryon_tail<5, C>(D, sn, scr, lane, kl);
   }
 }
 
@@ -160,7 +184,7 @@ __global__ __launch_bounds__(WAVE) void baryon_site_kernel(BaryonSiteArgs a)
   persistent_tiles<3>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
     C D[9];
     diquark_tensor(D, lds + g * SL, lds + (SITES_PER_WAVE + g) * SL, q, a.gij);
-    const C R = baryon_tail<T, C>(D, lds + (2 * SITES_PER_WAVE + g) * SL, lane, kl);
+    const C R = baryon_tail<T, C, false>(D, lds + (2 * SITES_PER_WAVE + g) * SL, lds + g * SL, lane, kl);
     const C c = group16_allreduce(cmul(w, R));
     if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
   });
@@ -252,11 +276,18 @@ __device__ __forceinline__ void project_accumulate(dbl4 (&acc_re)[NT], dbl4 (&ac
   }
 }
 
-// Copy the phase tables of the three spatial directions into LDS.
+// Build the phase tables exp(-2 pi i k / L_mu) of the three spatial directions
+// in LDS ([Lx | Ly | Lz] entries); a few sincospi per lane per work item.
 __device__ __forceinline__ void load_phase_tab(double2 *tab, const ProjectArgs &pa, int lane)
 {
   const int n = pa.dims[0] + pa.dims[1] + pa.dims[2];
-  for (int e = lane; e < n; e += WAVE) tab[e] = pa.phase_tab[e];
+  for (int e = lane; e < n; e += WAVE) {
+    const int L = e < pa.dims[0] ? pa.dims[0] : (e < pa.dims[0] + pa.dims[1] ? pa.dims[1] : pa.dims[2]);
+    const int k = e < pa.dims[0] ? e : (e < pa.dims[0] + pa.dims[1] ? e - pa.dims[0] : e - pa.dims[0] - pa.dims[1]);
+    double sn, cs;
+    sincospi(2.0 * (double)k / (double)L, &sn, &cs);
+    tab[e] = make_double2(cs, -sn);
+  }
 }
 
 template <int NT>
@@ -365,7 +396,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT
     diquark_tensor(D, si, sj, q, a.gij);
     double2 R = make_double2(0.0, 0.0);
     for (int t = 0; t < a.nterm; ++t) {
-      const C r = baryon_tail_rt<C>(a.types[t], D, sn, lane, kl);
+      const C r = baryon_tail_rt<C>(a.types[t], D, sn, si, lane, kl);
       R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)r.x, (double)r.y));
     }
     if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));

@@ -38,11 +38,10 @@ struct ProjectArgs {
   int order;   // 0 lexicographic [t][z][y][x]; 1 even-odd [parity][t][z][y][x/2]
   int np;
   int mom[MAX_MOM][3];
-  const double2 *phase_tab; // exp(-2 pi i k / L_mu), [Lx | Ly | Lz]
-  int chunk;                // sites per work item (multiple of 4)
+  int chunk;   // sites per work item (multiple of 4)
   int chunks_per_slice;
-  size_t slice_sites;       // sites per (parity, t) slice
-  double2 *partial;         // [items][np][16]
+  size_t slice_sites; // sites per (parity, t) slice
+  double2 *partial;   // [items][np][16]
 };
 
 struct BaryonProjectArgs {
@@ -57,18 +56,26 @@ void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, si
 // by exact powers of two, FP64 projection and sums.
 void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream);
 
-// partial[items][np][16] -> out[w][p][t] = sum_q W_w[q] * sum_{items of t} partial
-struct FinishArgs {
+// partial[items][np][nq] -> open[t][np][nq]: per-timeslice sum over the
+// timeslice's work items (both parities), fixed order, Neumaier compensated.
+struct ReduceArgs {
   const double2 *partial;
-  int np, lt, order;
+  double2 *open;
+  int np, lt, nq, order;
   int chunks_per_slice;
-  int nproj;      // projectors in this launch (<= MAX_PROJ_PER_LAUNCH)
-  int proj_base;  // index of the first one in the output
-  double w_re[MAX_PROJ_PER_LAUNCH][16], w_im[MAX_PROJ_PER_LAUNCH][16];
-  double2 *out;   // [nproj_total][np][lt]
 };
-void launch_project_finish(const FinishArgs &f, hipStream_t stream);
-void launch_phase_table(double2 *tab, const int dims[3], hipStream_t stream);
+void launch_project_reduce(const ReduceArgs &r, hipStream_t stream);
+
+// out[w][p][t] = sum_q W_w[q] open[t][p][q] for up to MAX_PROJ_PER_LAUNCH projectors.
+struct ApplyArgs {
+  const double2 *open;
+  int np, lt, nq;
+  int nproj;     // projectors in this launch
+  int proj_base; // index of the first one in the output
+  double w_re[MAX_PROJ_PER_LAUNCH][16], w_im[MAX_PROJ_PER_LAUNCH][16];
+  double2 *out;  // [nproj_total][np][lt]
+};
+void launch_project_apply(const ApplyArgs &f, hipStream_t stream);
 
 // ---- meson --------------------------------------------------------------
 struct MesonArgs {

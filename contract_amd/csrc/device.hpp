@@ -19,6 +19,18 @@ namespace contract_amd
 #define CONTRACT_COF_STREAM 1
 #endif
 
+// Scheduling fence: keeps the compiler from hoisting the next group's LDS
+// loads above the current group's arithmetic, which bounds the registers a
+// wave holds at once (needed to run two FP64 waves per SIMD).
+#ifndef CONTRACT_SCHED_FENCES
+#define CONTRACT_SCHED_FENCES 1
+#endif
+#if CONTRACT_SCHED_FENCES
+#define CONTRACT_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define CONTRACT_SCHED_FENCE() ((void)0)
+#endif
+
 constexpr int NS = 4;
 constexpr int NC = 3;
 constexpr int SITE_ELEMS = 144; // [4][4][3][3] complex entries per site
@@ -161,12 +173,12 @@ template <typename C> __device__ __forceinline__ C group16_reduce_scatter(C (&v)
 // site is padded by one 16-byte chunk so the four sites of a tile start on
 // different banks.  Sites past `nsites` (ragged tail) are zero-filled.
 // ---------------------------------------------------------------------------
-template <typename C> struct Tile {
+template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
   static constexpr int SITE16 = SITE_ELEMS * (int)sizeof(C) / 16;     // 144 (FP64) / 72 (FP32)
   static constexpr int LDS16 = SITE16 + 1;                             // padded site stride
   static constexpr int LDS_ELEMS = LDS16 * 16 / (int)sizeof(C);       // per site, in complex
-  static constexpr int CHUNKS = SITES_PER_WAVE * SITE16;              // per tile
-  static constexpr int PER_LANE = (CHUNKS + WAVE - 1) / WAVE;         // 9 / 5
+  static constexpr int CHUNKS = NSITE * SITE16;                       // per tile
+  static constexpr int PER_LANE = (CHUNKS + WAVE - 1) / WAVE;         // 4 sites: 9 / 5; 2 sites: 5 / 3
   uint4 v[PER_LANE];
 
   __device__ __forceinline__ void load(const C *__restrict__ src, size_t first_site, int nsites, int lane)
@@ -176,7 +188,8 @@ template <typename C> struct Tile {
 #pragma unroll
     for (int it = 0; it < PER_LANE; ++it) {
       const int e = it * WAVE + lane;
-      v[it] = (e < total) ? base[e] : make_uint4(0, 0, 0, 0);
+      if (CHUNKS % WAVE == 0 || it + 1 < PER_LANE || e < CHUNKS) // skip lanes past the tile
+        v[it] = (e < total) ? base[e] : make_uint4(0, 0, 0, 0);
     }
   }
 

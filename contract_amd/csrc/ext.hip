@@ -69,9 +69,9 @@ ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_i
   p.chunk = (int)chunk;
   p.chunks_per_slice = (int)((p.slice_sites + chunk - 1) / chunk);
   p.items = (size_t)p.slices * p.chunks_per_slice;
-  p.tab_bytes = align256(sizeof(double2) * (dims[0] + dims[1] + dims[2]));
   p.partial_bytes = align256(p.items * (size_t)nmom * 16 * sizeof(double2));
-  p.total = p.tab_bytes + p.partial_bytes;
+  p.open_bytes = align256((size_t)dims[3] * nmom * 16 * sizeof(double2));
+  p.total = p.partial_bytes + p.open_bytes;
   return p;
 }
 
@@ -115,25 +115,33 @@ void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const int dims[
   pa.np = nmom;
   for (int p = 0; p < nmom; ++p)
     for (int mu = 0; mu < 3; ++mu) pa.mom[p][mu] = momenta[3 * p + mu];
-  pa.phase_tab = reinterpret_cast<const double2 *>(ws);
   pa.chunk = plan.chunk;
   pa.chunks_per_slice = plan.chunks_per_slice;
   pa.slice_sites = plan.slice_sites;
-  pa.partial = reinterpret_cast<double2 *>(static_cast<char *>(ws) + plan.tab_bytes);
+  pa.partial = reinterpret_cast<double2 *>(ws);
 }
 
-// Launch the finishing reductions for nproj projectors (8 per launch).
+// Per-timeslice reduction of the work-item partials, then the projectors
+// (MAX_PROJ_PER_LAUNCH per launch).
 void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const zhost *W, int nproj, void *out,
-                       hipStream_t stream)
+                       void *ws, hipStream_t stream)
 {
+  ReduceArgs r;
+  r.partial = pa.partial;
+  r.open = reinterpret_cast<double2 *>(static_cast<char *>(ws) + plan.partial_bytes);
+  r.np = pa.np;
+  r.lt = pa.dims[3];
+  r.nq = 16;
+  r.order = pa.order;
+  r.chunks_per_slice = plan.chunks_per_slice;
+  launch_project_reduce(r, stream);
   for (int base = 0; base < nproj; base += MAX_PROJ_PER_LAUNCH) {
-    FinishArgs f;
+    ApplyArgs f;
     std::memset(&f, 0, sizeof(f));
-    f.partial = pa.partial;
+    f.open = r.open;
     f.np = pa.np;
     f.lt = pa.dims[3];
-    f.order = pa.order;
-    f.chunks_per_slice = plan.chunks_per_slice;
+    f.nq = 16;
     f.nproj = nproj - base < MAX_PROJ_PER_LAUNCH ? nproj - base : MAX_PROJ_PER_LAUNCH;
     f.proj_base = base;
     for (int w = 0; w < f.nproj; ++w)
@@ -142,7 +150,7 @@ void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const zho
         f.w_im[w][q] = W[(base + w) * 16 + q].imag();
       }
     f.out = static_cast<double2 *>(out);
-    launch_project_finish(f, stream);
+    launch_project_apply(f, stream);
   }
 }
 
@@ -238,7 +246,6 @@ int contract_baryon_twopoint_project(void *out, const void *propag_i, const void
 
   ProjectArgs pa;
   fill_project_args(pa, plan, dims, site_order, momenta, nmom, ws);
-  launch_phase_table(const_cast<double2 *>(pa.phase_tab), dims, stream);
 
   BaryonProjectArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -266,7 +273,7 @@ int contract_baryon_twopoint_project(void *out, const void *propag_i, const void
   std::vector<zhost> W((size_t)nproj * 16);
   const double *pm = reinterpret_cast<const double *>(projectors);
   for (size_t e = 0; e < W.size(); ++e) W[e] = zhost(pm[2 * e], pm[2 * e + 1]);
-  launch_finish_all(pa, plan, W.data(), nproj, out, stream);
+  launch_finish_all(pa, plan, W.data(), nproj, out, ws, stream);
   CONTRACT_EXT_HIP(hipGetLastError());
   if (own) CONTRACT_EXT_HIP(hipFreeAsync(ws, stream));
   return CONTRACT_OK;

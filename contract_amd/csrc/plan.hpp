@@ -4,22 +4,23 @@
 #include <stddef.h>
 
 #include "baryon.hpp"
+#include "spin.hpp"
 
 namespace contract_amd
 {
 
 // A work item is a contiguous run of `chunk` sites inside one slice (a
 // timeslice, or a timeslice of one parity for even-odd order), so every item
-// contributes to exactly one timeslice.  Workspace = phase table + one
-// [nmom][16] complex partial per item.
+// contributes to exactly one timeslice.  Workspace = one [nmom][16] complex
+// partial per item + the [Lt][nmom][16] per-timeslice sums.
 struct ProjectPlan {
   int slices;
   size_t slice_sites;
   int chunk;
   int chunks_per_slice;
   size_t items;
-  size_t tab_bytes;
-  size_t partial_bytes;
+  size_t partial_bytes; // [items][nmom][16] complex
+  size_t open_bytes;    // [Lt][nmom][16] complex, the per-timeslice sums
   size_t total;
 };
 
@@ -29,5 +30,7 @@ int check_lattice(const int dims[4], int order);
 int check_momenta(const int *momenta, int nmom);
 void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const int dims[4], int order, const int *momenta,
                        int nmom, void *ws);
+void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const zhost *W, int nproj, void *out,
+                       void *ws, hipStream_t stream);
 
 } // namespace contract_amd

@@ -1,39 +1,11 @@
-// project.hip -- phase tables and the deterministic per-timeslice reduction
-// that closes every fused (timeslice x momentum) contraction.
+// project.hip -- the deterministic per-timeslice reduction and projector
+// application that close every fused (timeslice x momentum) contraction.
 
 #include "baryon.hpp"
 #include "device.hpp"
 
 namespace contract_amd
 {
-
-// tab = [exp(-2 pi i k/Lx), k < Lx | same for Ly | same for Lz]
-__global__ void phase_table_kernel(double2 *tab, int lx, int ly, int lz)
-{
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int L, k;
-  if (e < lx) {
-    L = lx;
-    k = e;
-  } else if (e < lx + ly) {
-    L = ly;
-    k = e - lx;
-  } else if (e < lx + ly + lz) {
-    L = lz;
-    k = e - lx - ly;
-  } else {
-    return;
-  }
-  double s, c;
-  sincospi(2.0 * (double)k / (double)L, &s, &c);
-  tab[e] = make_double2(c, -s);
-}
-
-void launch_phase_table(double2 *tab, const int dims[3], hipStream_t stream)
-{
-  const int n = dims[0] + dims[1] + dims[2];
-  hipLaunchKernelGGL(phase_table_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, tab, dims[0], dims[1], dims[2]);
-}
 
 __device__ __forceinline__ void neumaier(double &sum, double &comp, double v)
 {
@@ -42,40 +14,69 @@ __device__ __forceinline__ void neumaier(double &sum, double &comp, double v)
   sum = t;
 }
 
-// One block per (momentum p, timeslice t).  Lanes 0..15 sum the work-item
-// partials of component q over the timeslice's items (both parities for
-// even-odd order) in item order, with Neumaier compensation; lanes then apply
-// the projectors W_w[q].
-__global__ __launch_bounds__(64) void project_finish_kernel(FinishArgs f)
+// One block of 256 threads per (momentum p, timeslice t): thread (c0, q)
+// sums component q over the work items c0, c0 + 16, ... of the timeslice
+// (both parities for even-odd order); the 16 column partials are then
+// combined in index order.  Every sum is Neumaier compensated and the order
+// is fixed, so results are bit-reproducible.
+__global__ __launch_bounds__(256) void project_reduce_kernel(ReduceArgs r)
 {
-  __shared__ double2 rp[16];
-  const int p = blockIdx.x, t = blockIdx.y, lane = threadIdx.x;
-  if (lane < 16) {
-    double sr = 0, cr = 0, si = 0, ci = 0;
-    const int nslices = f.order ? 2 : 1;
+  __shared__ double part[16][16][4];
+  const int p = blockIdx.x, t = blockIdx.y;
+  const int q = threadIdx.x & 15, c0 = threadIdx.x >> 4;
+  double sr = 0, cr = 0, si = 0, ci = 0;
+  if (q < r.nq) {
+    const int nslices = r.order ? 2 : 1;
     for (int par = 0; par < nslices; ++par) {
-      const size_t slice = (size_t)par * f.lt + t;
-      for (int c = 0; c < f.chunks_per_slice; ++c) {
-        const size_t item = slice * f.chunks_per_slice + c;
-        const double2 v = f.partial[(item * f.np + p) * 16 + lane];
+      const size_t slice = (size_t)par * r.lt + t;
+      for (int c = c0; c < r.chunks_per_slice; c += 16) {
+        const size_t item = slice * r.chunks_per_slice + c;
+        const double2 v = r.partial[(item * r.np + p) * r.nq + q];
         neumaier(sr, cr, v.x);
         neumaier(si, ci, v.y);
       }
     }
-    rp[lane] = make_double2(sr + cr, si + ci);
   }
+  part[c0][q][0] = sr;
+  part[c0][q][1] = cr;
+  part[c0][q][2] = si;
+  part[c0][q][3] = ci;
   __syncthreads();
-  for (int w = lane; w < f.nproj; w += 64) {
-    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int q = 0; q < 16; ++q) acc = cfma(acc, make_double2(f.w_re[w][q], f.w_im[w][q]), rp[q]);
-    f.out[((size_t)(f.proj_base + w) * f.np + p) * f.lt + t] = acc;
+  if (c0 == 0 && q < r.nq) {
+    double Sr = 0, Cr = 0, Si = 0, Ci = 0;
+    for (int c = 0; c < 16; ++c) {
+      neumaier(Sr, Cr, part[c][q][0]);
+      Cr += part[c][q][1];
+      neumaier(Si, Ci, part[c][q][2]);
+      Ci += part[c][q][3];
+    }
+    r.open[((size_t)t * r.np + p) * r.nq + q] = make_double2(Sr + Cr, Si + Ci);
   }
 }
 
-void launch_project_finish(const FinishArgs &f, hipStream_t stream)
+void launch_project_reduce(const ReduceArgs &r, hipStream_t stream)
 {
-  hipLaunchKernelGGL(project_finish_kernel, dim3(f.np, f.lt), dim3(64), 0, stream, f);
+  hipLaunchKernelGGL(project_reduce_kernel, dim3(r.np, r.lt), dim3(256), 0, stream, r);
+}
+
+__global__ __launch_bounds__(256) void project_apply_kernel(ApplyArgs f)
+{
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; // (w, p, t)
+  const size_t total = (size_t)f.nproj * f.np * f.lt;
+  if (e >= total) return;
+  const int t = (int)(e % f.lt);
+  const int p = (int)((e / f.lt) % f.np);
+  const int w = (int)(e / ((size_t)f.lt * f.np));
+  const double2 *o = f.open + ((size_t)t * f.np + p) * f.nq;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int q = 0; q < f.nq; ++q) acc = cfma(acc, make_double2(f.w_re[w][q], f.w_im[w][q]), o[q]);
+  f.out[((size_t)(f.proj_base + w) * f.np + p) * f.lt + t] = acc;
+}
+
+void launch_project_apply(const ApplyArgs &f, hipStream_t stream)
+{
+  const size_t total = (size_t)f.nproj * f.np * f.lt;
+  hipLaunchKernelGGL(project_apply_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, f);
 }
 
 } // namespace contract_amd
