@@ -158,12 +158,70 @@ def cpu_baseline(torch, props, gpu_fn, target_s=10.0):
     }
 
 
+def precision_study(torch, pc, dev, steps, warmup, hip, timer):
+    """Sequential-source precision study: 32^3x64, propagators decaying as
+    exp(-0.8|t|) (~20 decades over the lattice), stored in FP32.  Two forward
+    propagators and a sequential propagator are contracted with Cg5 and all 16
+    insertions at p = 0; the per-timeslice sum of (sequential source x
+    sequential propagator) equals this two-point contraction with the
+    sequential propagator in the n slot (sum_x P_n . dC/dP_n = C).  FP32
+    storage + FP32 contraction + FP64 accumulation vs the FP64 oracle run on
+    the same FP32-stored values."""
+    import oracle
+    dims = (32, 32, 32, 64)
+    V = 32 ** 3 * 64
+    terms = [(0, 1.0)]
+    mom = [(0, 0, 0)]
+    f32 = [p.to(torch.complex64) for p in make_propagators(torch, dims, (2001, 2002, 2003), 0.8, dev)]
+    torch.cuda.synchronize()
+    out32 = torch.empty((16, 1, 64), dtype=torch.complex128, device=dev)
+    ws = torch.empty(pc.project_workspace_bytes(dims, 1), dtype=torch.uint8, device=dev)
+
+    def step32():
+        pc.baryonTwoPointProject(*f32, terms, GAMMA_CG5, GAMMA_CG5, list(range(16)), mom, dims, out=out32,
+                                 workspace=ws)
+
+    wall32, kern32 = time_steps(torch, step32, steps, warmup, hip, timer)
+    got = out32.cpu().numpy()
+    f64 = [p.to(torch.complex128) for p in f32]
+    out64 = torch.empty_like(out32)
+
+    def step64():
+        pc.baryonTwoPointProject(*f64, terms, GAMMA_CG5, GAMMA_CG5, list(range(16)), mom, dims, out=out64,
+                                 workspace=ws)
+
+    wall64, kern64 = time_steps(torch, step64, steps, warmup, hip, timer)
+    host = [p.cpu().numpy() for p in f64]
+    del f64
+    t0 = time.perf_counter()
+    want = oracle.baryon_twopoint_project(*host, terms, GAMMA_CG5, GAMMA_CG5, list(range(16)), dims, mom)
+    t_oracle = time.perf_counter() - t0
+    per_t = np.abs(got - want).max(axis=(0, 1)) / np.abs(want).max(axis=(0, 1))
+    return {
+        "workload": "sequential-source precision study, 32^3x64, exp(-0.8|t|), FP32 storage, Cg5, 16 insertions, p=0",
+        "max_per_timeslice_rel_err": float(per_t.max()),
+        "median_per_timeslice_rel_err": float(np.median(per_t)),
+        "dynamic_range_decades": float(np.log10(np.abs(want[:, 0, 0]).max() / np.abs(want[:, 0, -1]).max())),
+        "fp64_gpu_vs_oracle_max_rel_err": float((np.abs(out64.cpu().numpy() - want).max(axis=(0, 1))
+                                                 / np.abs(want).max(axis=(0, 1))).max()),
+        "fp32_kernel_ms": kern32,
+        "fp64_kernel_ms": kern64,
+        "fp32_sites_per_s": V / (wall32 * 1e-3),
+        "fp64_sites_per_s": V / (wall64 * 1e-3),
+        "fp32_speedup": wall64 / wall32,
+        "fp32_achieved_GBs": BYTES_PER_SITE_FP32 * V / (kern32 * 1e-3) / 1e9,
+        "fp64_achieved_GBs": BYTES_PER_SITE_FP64 * V / (kern64 * 1e-3) / 1e9,
+        "oracle_s": t_oracle,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-precision", action="store_true", help="skip the 32^3x64 FP32 precision study")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 2 GiB device copy, the reference kernel for scaling HBM counters")
     args = ap.parse_args()
@@ -211,7 +269,6 @@ def main():
 
     wall32, kern32 = time_steps(torch, step32, args.steps, args.warmup, hip, timer)
     err32 = per_timeslice_error(out32.cpu().numpy(), ref64)
-    del u_f, d_f, u2_f
 
     # drop-in per-site kernel (one insertion, one term), reference-style output
     correl = torch.empty(V, dtype=torch.complex128, device=dev)
@@ -220,6 +277,12 @@ def main():
         pc.baryon_two_point(correl, u, d, u2, 5, V, GAMMA_CG5, GAMMA_CG5, 8)
 
     wall_abi, _ = time_steps(torch, step_abi, args.steps, args.warmup)
+
+    del u_f, d_f, u2_f
+    precision = None
+    if not args.no_precision:
+        del ws
+        precision = precision_study(torch, pc, dev, args.steps, args.warmup, hip, timer)
 
     cpu = None
     if not args.no_cpu:
@@ -275,6 +338,7 @@ def main():
             "achieved_GBs": (BYTES_PER_SITE_FP64 + 16) * V / (wall_abi * 1e-3) / 1e9,
             "note": "one type (IM_JL_NK), one insertion, per-site output, like one reference call",
         },
+        "precision_study": precision,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))
