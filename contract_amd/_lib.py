@@ -33,6 +33,8 @@ EXT_SYMBOLS = (
     "contract_project_workspace_bytes",
     "contract_baryon_twopoint_project",
     "contract_set_kernel_timer",
+    "contract_meson_grid_workspace_bytes",
+    "contract_meson_grid_project",
 )
 
 
@@ -77,6 +79,10 @@ def lib() -> ctypes.CDLL:
     L.contract_baryon_twopoint_project.restype = I
     L.contract_set_kernel_timer.argtypes = [P, P]
     L.contract_set_kernel_timer.restype = I
+    L.contract_meson_grid_workspace_bytes.argtypes = [P, I, I, P]
+    L.contract_meson_grid_workspace_bytes.restype = S
+    L.contract_meson_grid_project.argtypes = [P, I, I, P, I, P, P, S, P]
+    L.contract_meson_grid_project.restype = I
     _lib = L
     return L
 

@@ -1,0 +1,57 @@
+// meson_grid.hpp -- batched meson Gamma x Gamma' grids projected to
+// timeslices and momenta (many requests, uneven lattices, one launch).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "device.hpp"
+
+namespace contract_amd
+{
+
+constexpr int GRID_THREADS = 256;       // four waves share one staged tile
+constexpr int GRID_MAX_REQ = 32;        // requests per launch
+constexpr int GRID_MOM_PER_PASS = 8;    // 16 MFMA rows = 8 momenta x (re, im)
+constexpr int GRID_COLS = 256;          // K[i,k,j,l] columns, (k*4 + l)*16 + i*4 + j
+constexpr int GRID_PARTIAL = 16 * 2 * GRID_COLS; // doubles per work item: [row][K re/im][col]
+
+struct MesonGridReq {
+  const void *propag_i, *propag_j;
+  int dims[4];
+  int order;            // 0 lexicographic, 1 even-odd
+  int chunks_per_slice; // work items per (parity, t) slice
+  unsigned slice_sites;
+  unsigned item0;       // first work item of this request in the batch
+};
+
+struct MesonGridArgs {
+  MesonGridReq req[GRID_MAX_REQ];
+  int nreq;
+  int np;    // momenta in this pass (<= GRID_MOM_PER_PASS)
+  int chunk; // sites per work item, a multiple of the tile
+  int mom[GRID_MOM_PER_PASS][3];
+  unsigned total_items;
+  double *partial; // [total_items][GRID_PARTIAL]
+};
+
+struct MesonGridOut {
+  void *out; // complex128 [16][16][np_total][Lt]
+  int lt, order, chunks_per_slice;
+  unsigned item0;
+  int block0; // first finish block (one per timeslice)
+};
+
+struct MesonGridFinish {
+  MesonGridOut req[GRID_MAX_REQ];
+  int nreq, np, np_total, p_base;
+  const double *partial;
+  SparseSpin g5t[16]; // (g5 G(g))^T: row i -> (j, (g5 G(g))[j][i])
+};
+
+// tile_sites: sites staged per step (4 or 8); blocks: persistent grid size.
+void launch_meson_grid(const MesonGridArgs &a, bool fp32, int tile_sites, size_t blocks, size_t tab_entries,
+                       hipStream_t stream);
+void launch_meson_grid_finish(const MesonGridFinish &f, int nblocks, hipStream_t stream);
+
+} // namespace contract_amd

@@ -254,3 +254,93 @@ def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, ga
     if rc != 0:
         raise RuntimeError(f"contract_baryon_twopoint_project failed ({rc}): {last_error()}")
     return out
+
+
+# ---------------------------------------------------------------------------
+# Batched meson Gamma x Gamma' grids (include/contract_ext.h)
+# ---------------------------------------------------------------------------
+
+class MesonRequest(ctypes.Structure):
+    """contract_meson_request."""
+    _fields_ = [("propag_i", ctypes.c_void_p), ("propag_j", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("dims", ctypes.c_int * 4), ("site_order", ctypes.c_int)]
+
+
+class MesonGridTuning(ctypes.Structure):
+    """contract_meson_grid_tuning; zero fields take the library default."""
+    _fields_ = [("sites_per_item", ctypes.c_int), ("tile_sites", ctypes.c_int), ("blocks_per_cu", ctypes.c_int)]
+
+    def key(self):
+        return (self.sites_per_item, self.tile_sites, self.blocks_per_cu)
+
+
+def _tuning(tuning):
+    if tuning is None:
+        return None
+    if isinstance(tuning, MesonGridTuning):
+        return tuning
+    return MesonGridTuning(*[int(v) for v in tuning])
+
+
+def _request_array(requests, outs):
+    arr = (MesonRequest * len(requests))()
+    for r, (req, out) in enumerate(zip(requests, outs)):
+        pi, pj, dims = req[0], req[1], req[2]
+        order = int(req[3]) if len(req) > 3 else LEXICOGRAPHIC
+        arr[r].propag_i = _ptr(pi)
+        arr[r].propag_j = _ptr(pj)
+        arr[r].out = _ptr(out)
+        for mu in range(4):
+            arr[r].dims[mu] = int(dims[mu])
+        arr[r].site_order = order
+    return arr
+
+
+def meson_grid_workspace_bytes(requests, nmom: int, tuning=None) -> int:
+    """Device scratch (bytes) of mesonGridProject for these requests; a request
+    is (propag_i, propag_j, dims[, site_order]), only dims/site_order are read."""
+    arr = (MesonRequest * len(requests))()
+    for r, req in enumerate(requests):
+        arr[r].propag_i = arr[r].propag_j = arr[r].out = 1
+        for mu in range(4):
+            arr[r].dims[mu] = int(req[2][mu])
+        arr[r].site_order = int(req[3]) if len(req) > 3 else LEXICOGRAPHIC
+    t = _tuning(tuning)
+    return int(lib().contract_meson_grid_workspace_bytes(arr, len(requests), int(nmom),
+                                                         ctypes.byref(t) if t is not None else None))
+
+
+def mesonGridProject(requests, momenta, tuning=None, outs=None, workspace=None, stream=None):
+    """Batched meson two-point grids: for each request (propag_i, propag_j,
+    dims[, site_order]) return out[16 Gamma_ij][16 Gamma_kl][len(momenta)][Lt]
+    (complex128 on the device) with out[g1][g2][p][t] = sum_{x in t}
+    exp(-i p.x) meson_two_point(propag_i, propag_j; g1, g2)(x).
+
+    All requests go through one persistent launch (per 32 requests and 8
+    momenta).  complex64 propagators select the FP32-contraction /
+    FP64-accumulation form; all requests must share the dtype."""
+    import torch
+    if not requests:
+        return []
+    dt = requests[0][0].dtype
+    for req in requests:
+        if req[0].dtype != dt or req[1].dtype != dt:
+            raise TypeError("all propagators of a batch must have the same dtype")
+        _volume(req[0])
+        _volume(req[1])
+    precision = CONTRACT_FP32 if dt == torch.complex64 else CONTRACT_FP64
+    mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
+    if outs is None:
+        outs = [torch.empty((16, 16, mom.shape[0], int(req[2][3])), dtype=torch.complex128, device=req[0].device)
+                for req in requests]
+    arr = _request_array(requests, outs)
+    t = _tuning(tuning)
+    ws_ptr, ws_bytes = 0, 0
+    if workspace is not None:
+        ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    rc = lib().contract_meson_grid_project(arr, len(requests), precision, mom.ctypes.data_as(ctypes.c_void_p),
+                                           mom.shape[0], ctypes.byref(t) if t is not None else None, ws_ptr,
+                                           ws_bytes, 0 if stream is None else int(stream))
+    if rc != 0:
+        raise RuntimeError(f"contract_meson_grid_project failed ({rc}): {last_error()}")
+    return outs

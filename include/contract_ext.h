@@ -67,6 +67,41 @@ int contract_baryon_twopoint_project(void *out, const void *propag_i, const void
                                      int nproj, const int *momenta, int nmom, const int dims[4], int site_order,
                                      void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- batched meson Gamma x Gamma' grids -------------------------------
+ * One request = two propagators on one lattice.  For every request the full
+ * 16 x 16 grid of meson_two_point(propag_i, propag_j, volume, g1, g2) (the
+ * reference's include/contract.h:16 per-site correlator, one call per gamma
+ * pair there) is summed per timeslice and projected to momenta:
+ *
+ *   out[g1][g2][p][t] = sum_{x in t} exp(-i 2pi p.x / L) meson(x; g1, g2),
+ *
+ * out being device complex128 [16][16][nmom][Lt] of that request.  All
+ * requests of one call share precision, momenta and the workspace, and go
+ * through one persistent launch per group of 32 requests and 8 momenta, so
+ * small and large lattices balance across the chip. */
+typedef struct {
+  const void *propag_i; /* device [V][4][4][3][3], complex128 or complex64 */
+  const void *propag_j;
+  void *out;            /* device complex128 [16][16][nmom][Lt] */
+  int dims[4];          /* Lx, Ly, Lz, Lt */
+  int site_order;       /* CONTRACT_LEXICOGRAPHIC or CONTRACT_EVEN_ODD */
+} contract_meson_request;
+
+/* Work decomposition; a zero field takes the library default. */
+typedef struct {
+  int sites_per_item; /* sites per work item (rounded up to the tile) */
+  int tile_sites;     /* sites staged per step: 4 or 8 */
+  int blocks_per_cu;  /* persistent workgroups per compute unit: 1..8 */
+} contract_meson_grid_tuning;
+
+size_t contract_meson_grid_workspace_bytes(const contract_meson_request *reqs, int nreq, int nmom,
+                                           const contract_meson_grid_tuning *tuning);
+
+/* The kernel timer (contract_set_kernel_timer) brackets the whole batch. */
+int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, int precision, const int *momenta,
+                                int nmom, const contract_meson_grid_tuning *tuning, void *workspace,
+                                size_t workspace_bytes, void *stream);
+
 /* Profiling hook: while set, every projected entry point called from this
  * thread records `start` (a hipEvent_t) on its stream right before its main
  * contraction kernel and `stop` right after it, so a caller can time that

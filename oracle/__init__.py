@@ -212,6 +212,21 @@ def baryon_twopoint_project(propag_i, propag_j, propag_n, terms, gamma_ij: int, 
     return out
 
 
+def meson_grid_project(propag_i, propag_j, dims, momenta, order: int = 0) -> np.ndarray:
+    """The batched meson grid of one request: out[g1][g2][p][t] = projection
+    of meson_two_point(propag_i, propag_j; gamma_ij = g1, gamma_kl = g2)
+    (include/contract.h:16, include/meson.cuh:32-50), one oracle call per
+    gamma pair, then the oracle's timeslice/momentum projection."""
+    pi, pj = _prop(propag_i), _prop(propag_j)
+    V = _volume(pi)
+    corr = np.empty((V, 256), np.complex128)
+    for g1 in range(16):
+        for g2 in range(16):
+            corr[:, g1 * 16 + g2] = meson_two_point(pi, pj, g1, g2)
+    out = project(corr, dims, momenta, order)
+    return out.reshape(16, 16, out.shape[1], out.shape[2])
+
+
 # ---------------------------------------------------------------------------
 # Synthetic propagators
 # ---------------------------------------------------------------------------

@@ -119,3 +119,38 @@ def test_extension_rejects_bad_arguments_with_codes():
     assert call(out=0) != 0
     assert L.contract_project_workspace_bytes(dims.ctypes.data, 0, 27) > 0
     assert L.contract_project_workspace_bytes(dims.ctypes.data, 0, 0) == 0
+
+
+def test_meson_grid_rejects_bad_arguments_with_codes():
+    import ctypes
+    import numpy as np
+    import contract_amd
+    from contract_amd import pycontract as pc
+    L = contract_amd.lib()
+    mom = np.zeros((1, 3), np.int32)
+
+    def call(dims=(4, 4, 4, 8), order=0, nreq=1, prec=0, nmom=1, tuning=None, ptrs=(1, 1, 1)):
+        arr = (pc.MesonRequest * max(nreq, 1))()
+        for r in range(max(nreq, 1)):
+            arr[r].propag_i, arr[r].propag_j, arr[r].out = ptrs
+            for mu in range(4):
+                arr[r].dims[mu] = dims[mu]
+            arr[r].site_order = order
+        t = pc.MesonGridTuning(*tuning) if tuning else None
+        return L.contract_meson_grid_project(arr, nreq, prec, mom.ctypes.data, nmom,
+                                             ctypes.byref(t) if t else None, None, 0, None)
+
+    assert call(nreq=0) != 0 and "nreq" in contract_amd.last_error()
+    assert call(prec=2) != 0 and "precision" in contract_amd.last_error()
+    assert call(nmom=0) != 0 and "nmom" in contract_amd.last_error()
+    assert call(tuning=(0, 6, 0)) != 0 and "tile_sites" in contract_amd.last_error()
+    assert call(tuning=(0, 0, 9)) != 0 and "blocks_per_cu" in contract_amd.last_error()
+    assert call(tuning=(-5, 0, 0)) != 0 and "sites_per_item" in contract_amd.last_error()
+    assert call(dims=(3, 4, 4, 8), order=1) != 0 and "even" in contract_amd.last_error()
+    assert call(dims=(200, 200, 4, 8)) != 0 and "exceeds" in contract_amd.last_error()
+    assert call(ptrs=(1, 0, 1)) != 0 and "NULL" in contract_amd.last_error()
+    reqs = [(None, None, (16, 16, 16, 32)), (None, None, (24, 24, 24, 48), 1)]
+    # 16^3 x 32: 32 slices x 4 items; 24^3 x 48 even-odd: 96 slices x 7 items (1024 sites per item)
+    assert pc.meson_grid_workspace_bytes(reqs, 7) == (32 * 4 + 96 * 7) * 16 * 2 * 256 * 8
+    assert pc.meson_grid_workspace_bytes(reqs, 7, (4096, 8, 1)) == (32 * 1 + 96 * 2) * 16 * 2 * 256 * 8
+    assert pc.meson_grid_workspace_bytes(reqs, 0) == 0
