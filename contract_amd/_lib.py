@@ -33,6 +33,7 @@ EXT_SYMBOLS = (
     "contract_project_workspace_bytes",
     "contract_baryon_twopoint_project",
     "contract_set_kernel_timer",
+    "contract_set_stream",
     "contract_meson_grid_workspace_bytes",
     "contract_meson_grid_project",
 )
@@ -79,6 +80,8 @@ def lib() -> ctypes.CDLL:
     L.contract_baryon_twopoint_project.restype = I
     L.contract_set_kernel_timer.argtypes = [P, P]
     L.contract_set_kernel_timer.restype = I
+    L.contract_set_stream.argtypes = [P]
+    L.contract_set_stream.restype = I
     L.contract_meson_grid_workspace_bytes.argtypes = [P, I, I, P]
     L.contract_meson_grid_workspace_bytes.restype = S
     L.contract_meson_grid_project.argtypes = [P, I, I, P, I, P, P, S, P]

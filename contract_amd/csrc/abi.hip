@@ -1,6 +1,7 @@
 // abi.hip -- the drop-in C ABI (include/contract.h).  Each entry point checks
 // its arguments the way the reference's launcher does, builds a by-value
-// argument block and launches one kernel on the null stream.  No entry point
+// argument block and launches one kernel on the null stream (or on the
+// calling thread's stream set by contract_set_stream).  No entry point
 // allocates, copies to __constant__ memory, or synchronises.
 
 #include <contract.h>
@@ -56,7 +57,7 @@ void meson_two_point(void *correl, void *propag_i, void *propag_j, unsigned long
   a.a = g5spT(gamma_ij);
   a.b = g5spT(gamma_kl);
   a.correl = correl;
-  launch_meson(a, 0);
+  launch_meson(a, abi_stream());
   check_launch();
 }
 
@@ -74,7 +75,7 @@ static void meson_all(void **correl, void *propag_i, void *propag_j, unsigned lo
     a.run[g] = g5spT(g);
     a.correl[g] = correl[g];
   }
-  launch_meson_all(a, 0);
+  launch_meson_all(a, abi_stream());
   check_launch();
 }
 
@@ -100,7 +101,7 @@ void baryon_diquark(void *diquark, void *propag_i, void *propag_j, size_t volume
   a.volume = volume;
   a.gij = sp(gamma_ij);
   a.gklT = spT(gamma_kl);
-  launch_diquark(a, 0);
+  launch_diquark(a, abi_stream());
   check_launch();
 }
 
@@ -120,7 +121,7 @@ static void baryon_site(void *correl, void *propag_i, void *propag_j, void *prop
     a.w_re[q] = W[q].real();
     a.w_im[q] = W[q].imag();
   }
-  launch_baryon_site(contract_type, a, 0);
+  launch_baryon_site(contract_type, a, abi_stream());
   check_launch();
 }
 
@@ -176,7 +177,7 @@ void baryon_sequential_two_point(void *propag_i, void *propag_j, void *propag_n,
   a.gklT = spT(gamma_kl);
   a.gmn = sp(gamma_mn);
   a.gmnT = spT(gamma_mn);
-  launch_sequential((int)contract_type, target, a, 0);
+  launch_sequential((int)contract_type, target, a, abi_stream());
   check_launch();
 }
 

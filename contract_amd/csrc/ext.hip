@@ -17,6 +17,9 @@ namespace contract_amd
 
 static thread_local char g_last_error[512] = "";
 static thread_local hipEvent_t g_timer_start = nullptr, g_timer_stop = nullptr;
+static thread_local hipStream_t g_abi_stream = nullptr;
+
+hipStream_t abi_stream() { return g_abi_stream; }
 
 hipError_t timer_begin(hipStream_t stream)
 {
@@ -161,6 +164,12 @@ using namespace contract_amd;
 extern "C" {
 
 const char *contract_last_error(void) { return g_last_error; }
+
+int contract_set_stream(void *stream)
+{
+  g_abi_stream = static_cast<hipStream_t>(stream);
+  return CONTRACT_OK;
+}
 
 int contract_set_kernel_timer(void *start, void *stop)
 {

@@ -208,7 +208,7 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
       a.np = std::min(nmom - p0, GRID_MOM_PER_PASS);
       for (int p = 0; p < a.np; ++p)
         for (int mu = 0; mu < 3; ++mu) a.mom[p][mu] = momenta[3 * (p0 + p) + mu];
-      launch_meson_grid(a, precision == CONTRACT_FP32, g.tile, grid, (size_t)tab, stream);
+      launch_meson_grid(a, precision == CONTRACT_FP32, g.tile, grid, (size_t)tab * a.np, stream);
       CONTRACT_EXT_HIP(hipGetLastError());
       f.np = a.np;
       f.p_base = p0;

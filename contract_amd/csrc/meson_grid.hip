@@ -29,52 +29,26 @@ namespace contract_amd
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void grid_coords(size_t s, const int (&dims)[4], int order, int &x, int &y, int &z)
-{
-  if (order == 0) {
-    x = (int)(s % dims[0]);
-    s /= dims[0];
-    y = (int)(s % dims[1]);
-    s /= dims[1];
-    z = (int)(s % dims[2]);
-  } else {
-    const size_t half = (size_t)dims[0] * dims[1] * dims[2] * dims[3] / 2;
-    const int parity = (int)(s / half);
-    size_t r = s - (size_t)parity * half;
-    const int lxh = dims[0] >> 1;
-    const int xh = (int)(r % lxh);
-    r /= lxh;
-    y = (int)(r % dims[1]);
-    r /= dims[1];
-    z = (int)(r % dims[2]);
-    const int t = (int)(r / dims[2]);
-    x = 2 * xh + ((parity + y + z + t) & 1);
-  }
-}
-
 __device__ __forceinline__ int nonneg_mod(int a, int L)
 {
   const int r = a % L;
   return r < 0 ? r + L : r;
 }
 
-// Scale one staged FP32 site (16 lanes, 9 entries each) by an exact power of
-// two so its largest entry lies in [1, 2); returns the exponent.  The meson
-// traces are bilinear, so K of the scaled blocks times 2^(e_i + e_j) is exact
-// up to FP32 rounding and never underflows.
-__device__ __forceinline__ int grid_normalise(float2 *site, int q)
+// FP32 storage: the X block a lane holds is rescaled in registers by an
+// exact power of two so its largest entry lies in [1, 2); the trace with the
+// stored Y block then stays far above FP32's underflow threshold (it is as
+// large as Y's entries), and the FP64 result is scaled back exactly.
+__device__ __forceinline__ int scale_block(float2 (&b)[9])
 {
-  float2 *p = site + q * 9;
   float mx = 0.0f;
 #pragma unroll
-  for (int e = 0; e < 9; ++e) mx = fmaxf(mx, fmaxf(fabsf(p[e].x), fabsf(p[e].y)));
-#pragma unroll
-  for (int m = 8; m >= 1; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m, 64));
+  for (int e = 0; e < 9; ++e) mx = fmaxf(mx, fmaxf(fabsf(b[e].x), fabsf(b[e].y)));
   int ex = (mx > 0.0f) ? ilogbf(mx) : 0;
   ex = ex < -126 ? -126 : (ex > 126 ? 126 : ex);
   const float sc = ldexpf(1.0f, -ex);
 #pragma unroll
-  for (int e = 0; e < 9; ++e) p[e] = make_float2(p[e].x * sc, p[e].y * sc);
+  for (int e = 0; e < 9; ++e) b[e] = make_float2(b[e].x * sc, b[e].y * sc);
   return ex;
 }
 
@@ -86,16 +60,32 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
   static_assert(NS % SITES_PER_WAVE == 0, "tile is a whole number of MFMA site steps");
   using C = cplx<Real>;
   using T = Tile<C>;
-  constexpr int SL = T::LDS_ELEMS;           // padded site stride (complex entries)
-  constexpr int CH = NS * T::SITE16;         // 16-byte chunks per propagator tile
-  constexpr int PER_THREAD = (2 * CH + GRID_THREADS - 1) / GRID_THREADS;
-  __shared__ C lds[2][NS * SL];
-  __shared__ int sexp[2][NS];
+  constexpr int SL = T::LDS_ELEMS;               // padded site stride (complex entries)
+  constexpr int CH = NS * T::SITE16;             // 16-byte chunks per propagator tile
+  constexpr int NCH = 2 * CH;                    // both propagators
+  constexpr int PER_THREAD = (NCH + GRID_THREADS - 1) / GRID_THREADS;
+  constexpr int TILE16 = NS * T::LDS16;          // one propagator tile in LDS, 16-byte units
+  __shared__ C lds[2][NS * SL];                  // [propagator]
   extern __shared__ double2 tab[];
   const int tid = threadIdx.x;
   const int w = tid >> 6, lane = tid & 63;
   const int g = lane >> 4, q = lane & 15;
   const int i = q >> 2, j = q & 3;
+
+  // this thread's chunks of a tile: offset from the tile's first site in the
+  // propagator, which propagator, and the LDS slot (the same every step)
+  int src_off[PER_THREAD], dst_off[PER_THREAD];
+  bool second[PER_THREAD];
+#pragma unroll
+  for (int it = 0; it < PER_THREAD; ++it) {
+    const int e = it * GRID_THREADS + tid;
+    second[it] = e >= CH;
+    const int ec = e - (second[it] ? CH : 0);
+    const int s = ec / T::SITE16;
+    src_off[it] = ec;
+    dst_off[it] = (second[it] ? TILE16 : 0) + s * T::LDS16 + (ec - s * T::SITE16);
+  }
+  auto active = [&](int it) { return NCH % GRID_THREADS == 0 || it + 1 < PER_THREAD || tid < NCH % GRID_THREADS; };
 
   for (size_t item = blockIdx.x; item < a.total_items; item += gridDim.x) {
     int r = 0; // request of this item (uniform over the workgroup)
@@ -110,20 +100,42 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
     last = last < s0 + R.slice_sites ? last : s0 + R.slice_sites;
 
     __syncthreads(); // the previous item's readers of tab are done
-    const int ntab = R.dims[0] + R.dims[1] + R.dims[2];
+    // per-momentum phase tables exp(-2 pi i p_mu k / L_mu), k < L_mu:
+    // [np][Lx] | [np][Ly] | [np][Lz], so a lane's phase needs no division
+    const int Lx = R.dims[0], Ly = R.dims[1], Lz = R.dims[2];
+    const int offy = a.np * Lx, offz = a.np * (Lx + Ly);
+    const int ntab = a.np * (Lx + Ly + Lz);
     for (int e = tid; e < ntab; e += GRID_THREADS) {
-      const int L = e < R.dims[0] ? R.dims[0] : (e < R.dims[0] + R.dims[1] ? R.dims[1] : R.dims[2]);
-      const int k = e < R.dims[0] ? e : (e < R.dims[0] + R.dims[1] ? e - R.dims[0] : e - R.dims[0] - R.dims[1]);
+      const int mu = e < offy ? 0 : (e < offz ? 1 : 2);
+      const int L = R.dims[mu];
+      const int rel = e - (mu == 0 ? 0 : (mu == 1 ? offy : offz));
+      const int p = rel / L, k = rel - p * L;
+      const int m = nonneg_mod(a.mom[p][mu], L);
       double sn, cs;
-      sincospi(2.0 * (double)k / (double)L, &sn, &cs);
-      tab[e] = make_double2(cs, -sn); // exp(-2 pi i k / L)
+      sincospi(2.0 * (double)((m * k) % L) / (double)L, &sn, &cs);
+      tab[e] = make_double2(cs, -sn);
     }
     // this lane's MFMA row: momentum q >> 1, phase part q & 1 (0 re, 1 im)
     const int pr = q >> 1;
     const bool live = pr < a.np;
-    const int mx = live ? nonneg_mod(a.mom[pr][0], R.dims[0]) : 0;
-    const int my = live ? nonneg_mod(a.mom[pr][1], R.dims[1]) : 0;
-    const int mz = live ? nonneg_mod(a.mom[pr][2], R.dims[2]) : 0;
+    const int prow = live ? pr : 0;
+    const double2 *tx = tab + prow * Lx, *ty = tab + offy + prow * Ly, *tz = tab + offz + prow * Lz;
+
+    // coordinates of the step's first site, advanced incrementally (uniform):
+    // a slice is [z][y][xr] with xr = x (lexicographic) or x / 2 (even-odd,
+    // where x = 2 xr + ((parity + y + z + t) & 1))
+    const int lt = R.dims[3];
+    const int par = (int)(slice / lt), tsl = (int)(slice - (size_t)par * lt);
+    const int Lr = R.order ? Lx >> 1 : Lx;
+    int x0, y0, z0;
+    {
+      const unsigned r0 = (unsigned)(c * (size_t)a.chunk);
+      x0 = (int)(r0 % (unsigned)Lr);
+      const unsigned rest = r0 / (unsigned)Lr;
+      y0 = (int)(rest % (unsigned)Ly);
+      z0 = (int)(rest / (unsigned)Ly);
+    }
+    const int pbase = par + tsl;
 
     dbl4 acc_re[4], acc_im[4]; // column tiles l = 0..3 of k = w; B = K re / K im
 #pragma unroll
@@ -134,60 +146,57 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
     const C *P0 = static_cast<const C *>(R.propag_i), *P1 = static_cast<const C *>(R.propag_j);
     uint4 buf[PER_THREAD];
     auto load = [&](size_t st) {
+      const uint4 *b0 = reinterpret_cast<const uint4 *>(P0 + st * SITE_ELEMS);
+      const uint4 *b1 = reinterpret_cast<const uint4 *>(P1 + st * SITE_ELEMS);
       const size_t left = last - st;
-      const int total = (left < (size_t)NS ? (int)left : NS) * T::SITE16;
+      if (left >= (size_t)NS) {
 #pragma unroll
-      for (int it = 0; it < PER_THREAD; ++it) {
-        const int e = it * GRID_THREADS + tid;
-        if ((2 * CH) % GRID_THREADS == 0 || e < 2 * CH) {
-          const int p = e >= CH, ec = e - p * CH;
-          const uint4 *base = reinterpret_cast<const uint4 *>((p ? P1 : P0) + st * SITE_ELEMS);
-          buf[it] = (ec < total) ? base[ec] : make_uint4(0, 0, 0, 0);
-        }
+        for (int it = 0; it < PER_THREAD; ++it)
+          if (active(it)) buf[it] = (second[it] ? b1 : b0)[src_off[it]];
+      } else {
+        const int total = (int)left * T::SITE16;
+#pragma unroll
+        for (int it = 0; it < PER_THREAD; ++it)
+          if (active(it)) buf[it] = src_off[it] < total ? (second[it] ? b1 : b0)[src_off[it]] : make_uint4(0, 0, 0, 0);
       }
     };
     auto store = [&]() {
+      uint4 *l16 = reinterpret_cast<uint4 *>(&lds[0][0]);
 #pragma unroll
-      for (int it = 0; it < PER_THREAD; ++it) {
-        const int e = it * GRID_THREADS + tid;
-        if ((2 * CH) % GRID_THREADS == 0 || e < 2 * CH) {
-          const int p = e >= CH, ec = e - p * CH;
-          const int s = ec / T::SITE16;
-          reinterpret_cast<uint4 *>(lds[p])[s * T::LDS16 + (ec - s * T::SITE16)] = buf[it];
-        }
-      }
+      for (int it = 0; it < PER_THREAD; ++it)
+        if (active(it)) l16[dst_off[it]] = buf[it];
     };
-    if (first < last) load(first);
+    load(first);
     for (size_t st = first; st < last; st += NS) {
-      __syncthreads(); // readers of the previous tile are done
+      __syncthreads(); // the readers of the previous tile are done
       store();
       if (st + NS < last) load(st + NS);
       __syncthreads();
-      if constexpr (sizeof(Real) == 4) {
-        for (int u = w; u < NS / 2; u += 4) { // (propagator, 4-site group) per wave
-          const int e1 = grid_normalise(reinterpret_cast<float2 *>(lds[u & 1]) + ((u >> 1) * 4 + g) * SL, q);
-          if (q == 0) sexp[u & 1][(u >> 1) * 4 + g] = e1;
-        }
-        __syncthreads();
-      }
 #pragma unroll
       for (int h = 0; h < NS / SITES_PER_WAVE; ++h) {
         const int ls = h * SITES_PER_WAVE + g; // this lane's site in the tile
-        // phase of this lane's MFMA row at its site (padded sites carry K = 0)
-        const size_t site = st + ls < last ? st + ls : st;
-        int x, y, z;
-        grid_coords(site, R.dims, R.order, x, y, z);
+        // coordinates of site st + ls (sites past the item's end carry K = 0;
+        // their coordinates only need to stay inside the tables)
+        int xr = x0 + ls, y = y0, z = z0;
+        while (xr >= Lr) {
+          xr -= Lr;
+          if (++y == Ly) {
+            y = 0;
+            ++z;
+          }
+        }
+        z = z < Lz ? z : 0;
+        const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
         double av = 0.0;
         if (live) {
-          const double2 ph = cmul(cmul(tab[(mx * x) % R.dims[0]], tab[R.dims[0] + (my * y) % R.dims[1]]),
-                                  tab[R.dims[0] + R.dims[1] + (mz * z) % R.dims[2]]);
+          const double2 ph = cmul(cmul(tx[x], ty[y]), tz[z]);
           av = (q & 1) ? ph.y : ph.x;
         }
         const C *si = lds[0] + ls * SL, *sj = lds[1] + ls * SL;
-        int ex = 0;
-        if constexpr (sizeof(Real) == 4) ex = sexp[0][ls] + sexp[1][ls];
         C xb[9], yb[9];
         load_block(xb, si, i, w);
+        int ex = 0;
+        if constexpr (sizeof(Real) == 4) ex = scale_block(xb);
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
           load_block(yb, sj, j, l);
@@ -201,6 +210,14 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
           }
           acc_re[l] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, kr, acc_re[l], 0, 0, 0);
           acc_im[l] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, ki, acc_im[l], 0, 0, 0);
+        }
+      }
+      x0 += NS;
+      while (x0 >= Lr) {
+        x0 -= Lr;
+        if (++y0 == Ly) {
+          y0 = 0;
+          ++z0;
         }
       }
     }

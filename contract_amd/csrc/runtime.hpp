@@ -59,6 +59,9 @@ enum : int {
 };
 
 void set_last_error(const char *fmt, ...);
+// Stream of the calling thread's drop-in ABI launches (contract_set_stream;
+// the null stream unless set).
+hipStream_t abi_stream();
 // Record the calling thread's kernel-timer events (contract_set_kernel_timer).
 hipError_t timer_begin(hipStream_t stream);
 hipError_t timer_end(hipStream_t stream);

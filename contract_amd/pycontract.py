@@ -344,3 +344,19 @@ def mesonGridProject(requests, momenta, tuning=None, outs=None, workspace=None, 
     if rc != 0:
         raise RuntimeError(f"contract_meson_grid_project failed ({rc}): {last_error()}")
     return outs
+
+
+class abi_stream:
+    """Context manager: drop-in entry points called from this thread launch on
+    ``stream`` (a torch stream, a raw handle or None for the null stream)."""
+
+    def __init__(self, stream):
+        self.handle = getattr(stream, "cuda_stream", stream) or 0
+
+    def __enter__(self):
+        lib().contract_set_stream(self.handle)
+        return self
+
+    def __exit__(self, *exc):
+        lib().contract_set_stream(None)
+        return False

@@ -102,6 +102,11 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
                                 int nmom, const contract_meson_grid_tuning *tuning, void *workspace,
                                 size_t workspace_bytes, void *stream);
 
+/* Stream of the drop-in entry points of include/contract.h called from this
+ * thread (the reference launches them on the null stream, which stays the
+ * default).  Lets a caller overlap reference-ABI work on its own streams. */
+int contract_set_stream(void *stream);
+
 /* Profiling hook: while set, every projected entry point called from this
  * thread records `start` (a hipEvent_t) on its stream right before its main
  * contraction kernel and `stop` right after it, so a caller can time that

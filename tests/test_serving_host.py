@@ -1,0 +1,101 @@
+"""Host logic of the serving loop (no GPU): the LRU pool's pinning, eviction
+and admission arithmetic, and the synthetic request stream."""
+
+import numpy as np
+import pytest
+
+from contract_amd.serving import KINDS, OutKey, PoolFull, PropagatorPool, PropKey, make_requests
+
+SMALL = (4, 4, 4, 8)  # 512 sites, 1,179,648 bytes per FP64 propagator
+NB = 512 * 2304
+
+
+def pool(cap_entries):
+    made = []
+
+    def materialise(key):
+        made.append(key)
+        return ("tensor", key)
+
+    return PropagatorPool(cap_entries * NB, materialise), made
+
+
+def test_lru_eviction_order_and_pins():
+    p, made = pool(3)
+    a, b, c, d = (PropKey(SMALL, i) for i in range(4))
+    for k in (a, b, c):
+        p.acquire(k)
+        p.release(k)
+    assert p.used == 3 * NB and made == [a, b, c]
+    p.acquire(a)  # hit: a becomes most recently used, pinned
+    assert p.hits == 1
+    p.acquire(d)  # evicts b, the least recently used unpinned entry
+    assert b not in p and a in p and c in p and d in p
+    assert p.evictions == 1
+    p.release(a)
+    p.release(d)
+    p.acquire(b)  # c is now the oldest unpinned
+    assert c not in p and made[-1] == b
+
+
+def test_pinned_entries_are_never_evicted():
+    p, _ = pool(2)
+    a, b, c = (PropKey(SMALL, i) for i in range(3))
+    p.acquire(a)
+    p.acquire(b)
+    assert not p.fits([c])
+    with pytest.raises(PoolFull):
+        p.acquire(c)
+    assert a in p and b in p
+    p.release(b)
+    assert p.fits([c]) and p.fits([a, c])
+    p.acquire(c)
+    assert b not in p
+
+
+def test_fits_counts_resident_inputs_and_outputs():
+    p, _ = pool(3)
+    a, b, c = (PropKey(SMALL, i) for i in range(3))
+    p.acquire(a)
+    p.release(a)
+    p.acquire(b)  # pinned
+    out = OutKey(99, SMALL)
+    # a is resident and unpinned: keeping it + one new output fits in 3
+    assert p.fits([a], [out])
+    # a, a new c and an output would need 4 entries with b pinned
+    assert not p.fits([a, c], [out])
+    p.insert(out, None)
+    assert p.pinned_bytes() == 2 * NB
+
+
+def test_entry_larger_than_cap_is_rejected():
+    p, _ = pool(1)
+    with pytest.raises(PoolFull):
+        p.acquire(PropKey((8, 8, 8, 8), 0))
+
+
+def test_request_stream_is_poisson_and_mixed():
+    reqs = make_requests(400.0, 5.0, seed=3, catalogue=4)
+    n = len(reqs)
+    assert abs(n - 2000) < 4 * np.sqrt(2000)
+    gaps = np.diff([0.0] + [r.arrival for r in reqs])
+    assert abs(gaps.mean() - 1 / 400) < 0.1 / 400 * 4
+    frac = {k: sum(r.kind == k for r in reqs) / n for k in KINDS}
+    assert abs(frac["meson"] - 0.5) < 0.04 and abs(frac["baryon"] - 0.3) < 0.04 and abs(frac["diquark"] - 0.2) < 0.04
+    for r in reqs:
+        assert r.dims in [(16, 16, 16, 32), (24, 24, 24, 64), (32, 32, 32, 64), (48, 48, 48, 96)]
+        assert all(k.dims == r.dims for k in r.props)
+        if r.kind == "baryon":
+            assert 1 <= len(r.gammas) <= 16 and len(set(r.gammas)) == len(r.gammas)
+            assert r.props[0] == r.props[2]
+        if r.kind != "diquark":
+            assert 1 <= len(r.momenta) <= 27 and len(set(r.momenta)) == len(r.momenta)
+    again = make_requests(400.0, 5.0, seed=3, catalogue=4)
+    assert [(r.kind, r.dims, r.props, r.gammas, r.momenta, r.arrival) for r in again] == \
+        [(r.kind, r.dims, r.props, r.gammas, r.momenta, r.arrival) for r in reqs]
+
+
+def test_catalogue_seeds_are_stable():
+    k = PropKey((16, 16, 16, 32), 2, t0=14)
+    assert k.seed == PropKey((16, 16, 16, 32), 2, t0=14).seed
+    assert k.seed != PropKey((16, 16, 16, 32), 3, t0=14).seed
