@@ -290,6 +290,21 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle):
     return res
 
 
+def serving_leg(torch, args):
+    """Serving loop: Poisson arrivals of mixed requests (50% meson grid, 30%
+    proton two-point, 20% diquark) on lattices drawn from 16^3x32, 24^3x64,
+    32^3x64 and 48^3x96, 1-16 gammas and 1-27 momenta each, against a
+    catalogue of 4 FP64 propagators per lattice held in an LRU pool capped at
+    200 GB; one stream of `serving_duration` seconds per offered rate."""
+    from contract_amd import serving
+    rates = [float(r) for r in args.serving_rates.split(",")]
+    res = serving.sweep(rates, args.serving_duration, seed=7, cap_gb=200.0, streams=4, max_inflight=8)
+    res["workload"] = ("Poisson arrivals, 50% meson / 30% baryon / 20% diquark, lattices 16^3x32, 24^3x64, "
+                       "32^3x64, 48^3x96, 1-16 gammas, 1-27 momenta, LRU pool 200 GB, 4 streams")
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +314,10 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the 32^3x64 FP32 precision study")
     ap.add_argument("--no-grid", action="store_true", help="skip the batched meson grid leg")
     ap.add_argument("--only-grid", action="store_true", help="run only the batched meson grid leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sweep")
+    ap.add_argument("--only-serving", action="store_true", help="run only the serving-loop sweep")
+    ap.add_argument("--serving-rates", default="10,25,50,100,200", help="offered request rates (req/s)")
+    ap.add_argument("--serving-duration", type=float, default=3.0, help="seconds of arrivals per rate")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 2 GiB device copy, the reference kernel for scaling HBM counters")
     args = ap.parse_args()
@@ -314,6 +333,10 @@ def main():
     pc.init(0)
     hip = HipEvents()
     timer = pc.lib().contract_set_kernel_timer
+
+    if args.only_serving:
+        print(json.dumps({"serving": serving_leg(torch, args)}))
+        return
 
     if args.only_grid:
         print(json.dumps({"meson_grid_batch": meson_grid_batch(torch, pc, dev, args.steps, args.warmup, hip, timer,
@@ -369,6 +392,11 @@ def main():
     grid = None
     if not args.no_grid:
         grid = meson_grid_batch(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer, not args.no_cpu)
+
+    serving = None
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        serving = serving_leg(torch, args)
 
     cpu = None
     if not args.no_cpu:
@@ -426,6 +454,7 @@ def main():
         },
         "precision_study": precision,
         "meson_grid_batch": grid,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))

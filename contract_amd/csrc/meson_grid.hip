@@ -261,53 +261,74 @@ __device__ __forceinline__ void nsum(double &s, double &c, double v)
   s = t;
 }
 
-// One block of 256 threads per (request, timeslice), one thread per K column:
-// sum the timeslice's work-item partials in item order (Neumaier compensated,
-// so the result is bit-reproducible), combine the (phase part, K part) rows
-// into the complex momentum-projected K, then apply the gamma pairs, thread
-// (g1, g2).  Output out[g1][g2][p_base + p][t].
-__global__ __launch_bounds__(256) void meson_grid_finish_kernel(MesonGridFinish f)
+// One block per (request, timeslice, momentum), 512 threads = two chunk
+// groups x one thread per K column: each thread sums every other work-item
+// partial of the timeslice in item order (Neumaier compensated), the two
+// groups are combined in a fixed order (so the result is bit-reproducible),
+// the (phase part, K part) rows are combined into the complex
+// momentum-projected K, and thread (g1, g2) applies the gamma pair.  Output
+// out[g1][g2][p_base + p][t].
+constexpr int FINISH_GROUPS = 2;
+
+__global__ __launch_bounds__(FINISH_GROUPS * GRID_COLS) void meson_grid_finish_kernel(MesonGridFinish f)
 {
-  __shared__ double2 kp[GRID_MOM_PER_PASS][GRID_COLS];
-  const int col = threadIdx.x;
+  __shared__ double part[FINISH_GROUPS][4][2][GRID_COLS];
+  __shared__ double2 kp[GRID_COLS];
+  const int col = threadIdx.x % GRID_COLS, cg = threadIdx.x / GRID_COLS;
+  const int p = blockIdx.y;
   int r = 0;
   while (r + 1 < f.nreq && (int)blockIdx.x >= f.req[r + 1].block0) ++r;
   const MesonGridOut &R = f.req[r];
   const int t = (int)blockIdx.x - R.block0;
   const int nslices = R.order ? 2 : 1;
-  for (int p = 0; p < f.np; ++p) {
-    double s[4] = {0, 0, 0, 0}, cc[4] = {0, 0, 0, 0};
-    for (int par = 0; par < nslices; ++par) {
-      const size_t slice = (size_t)par * R.lt + t;
-      for (int ch = 0; ch < R.chunks_per_slice; ++ch) {
-        const double *src = f.partial + (R.item0 + slice * R.chunks_per_slice + ch) * (size_t)GRID_PARTIAL;
-        // row 2p: phase re, row 2p + 1: phase im; each with K re / K im
+  double s[4] = {0, 0, 0, 0}, cc[4] = {0, 0, 0, 0};
+  for (int par = 0; par < nslices; ++par) {
+    const size_t slice = (size_t)par * R.lt + t;
+    for (int ch = cg; ch < R.chunks_per_slice; ch += FINISH_GROUPS) {
+      const double *src = f.partial + (R.item0 + slice * R.chunks_per_slice + ch) * (size_t)GRID_PARTIAL;
+      // row 2p: phase re, row 2p + 1: phase im; each with K re / K im
 #pragma unroll
-        for (int u = 0; u < 4; ++u) nsum(s[u], cc[u], src[(4 * p + u) * GRID_COLS + col]);
-      }
+      for (int u = 0; u < 4; ++u) nsum(s[u], cc[u], src[(4 * p + u) * GRID_COLS + col]);
     }
-    const double rr = s[0] + cc[0], ri = s[1] + cc[1], ir = s[2] + cc[2], ii = s[3] + cc[3];
-    kp[p][col] = make_double2(rr - ii, ri + ir); // (ph_re + i ph_im)(K_re + i K_im)
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    part[cg][u][0][col] = s[u];
+    part[cg][u][1][col] = cc[u];
   }
   __syncthreads();
-  const int g1 = col >> 4, g2 = col & 15;
-  const SparseSpin a1 = f.g5t[g1], a2 = f.g5t[g2];
-  double2 *out = static_cast<double2 *>(R.out);
-  for (int p = 0; p < f.np; ++p) {
+  if (cg == 0) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double S = 0, Cc = 0;
+      for (int gq = 0; gq < FINISH_GROUPS; ++gq) {
+        nsum(S, Cc, part[gq][u][0][col]);
+        Cc += part[gq][u][1][col];
+      }
+      v[u] = S + Cc;
+    }
+    kp[col] = make_double2(v[0] - v[3], v[1] + v[2]); // (ph_re + i ph_im)(K_re + i K_im)
+  }
+  __syncthreads();
+  if (cg == 0) {
+    const int g1 = col >> 4, g2 = col & 15;
+    const SparseSpin a1 = f.g5t[g1], a2 = f.g5t[g2];
     double2 acc = make_double2(0.0, 0.0);
     for (int ii = 0; ii < 4; ++ii)
       for (int l = 0; l < 4; ++l) {
         const int jj = a1.col[ii], kk = a2.col[l];
         const double2 wgt = cmul(make_double2(a1.re[ii], a1.im[ii]), make_double2(a2.re[l], a2.im[l]));
-        acc = cfma(acc, wgt, kp[p][(kk * 4 + l) * 16 + ii * 4 + jj]);
+        acc = cfma(acc, wgt, kp[(kk * 4 + l) * 16 + ii * 4 + jj]);
       }
+    double2 *out = static_cast<double2 *>(R.out);
     out[((size_t)(g1 * 16 + g2) * f.np_total + f.p_base + p) * R.lt + t] = acc;
   }
 }
 
 void launch_meson_grid_finish(const MesonGridFinish &f, int nblocks, hipStream_t stream)
 {
-  hipLaunchKernelGGL(meson_grid_finish_kernel, dim3(nblocks), dim3(256), 0, stream, f);
+  hipLaunchKernelGGL(meson_grid_finish_kernel, dim3(nblocks, f.np), dim3(FINISH_GROUPS * GRID_COLS), 0, stream, f);
 }
 
 } // namespace contract_amd

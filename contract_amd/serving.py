@@ -135,9 +135,14 @@ class PropagatorPool:
         kept = sum(self.entries[k][1] for k in set(keys) if k in self.entries and self.entries[k][2] == 0)
         return self.pinned_bytes() + kept + need <= self.cap
 
-    def _make_room(self, nbytes: int) -> None:
+    def _make_room(self, nbytes: int):
+        """Evict least-recently-used unpinned entries until `nbytes` fit.
+        Returns the value of an evicted entry of exactly `nbytes` (its memory
+        can be reused, which spares the allocator a free and a fresh
+        allocation) or None."""
         if nbytes > self.cap:
             raise PoolFull(f"entry of {nbytes} bytes exceeds the pool cap of {self.cap}")
+        recycled = None
         for key in list(self.entries):
             if self.used + nbytes <= self.cap:
                 break
@@ -146,8 +151,11 @@ class PropagatorPool:
                 del self.entries[key]
                 self.used -= size
                 self.evictions += 1
+                if recycled is None and size == nbytes and value is not None:
+                    recycled = value
         if self.used + nbytes > self.cap:
             raise PoolFull(f"no room for {nbytes} bytes: {self.used} used, {self.pinned_bytes()} pinned")
+        return recycled
 
     def acquire(self, key):
         e = self.entries.get(key)
@@ -157,19 +165,21 @@ class PropagatorPool:
             e[2] += 1
             return e[0]
         self.misses += 1
-        self._make_room(key.nbytes)
-        value = self.materialise(key)
+        recycled = self._make_room(key.nbytes)
+        value = self.materialise(key, recycled)
         self.entries[key] = [value, key.nbytes, 1]
         self.used += key.nbytes
         self.peak = max(self.peak, self.used)
         return value
 
-    def insert(self, key, value) -> None:
-        """Add a new pinned entry (a result)."""
-        self._make_room(key.nbytes)
+    def insert(self, key, value):
+        """Add a new pinned entry (a result, or None to reserve its bytes);
+        returns a recycled value of the same size, or None."""
+        recycled = self._make_room(key.nbytes)
         self.entries[key] = [value, key.nbytes, 1]
         self.used += key.nbytes
         self.peak = max(self.peak, self.used)
+        return recycled
 
     def get(self, key):
         return self.entries[key][0]
@@ -187,19 +197,21 @@ class PropagatorPool:
 # GPU side
 # ---------------------------------------------------------------------------
 
-def device_propagator(torch, key: PropKey, device):
+def device_propagator(torch, key: PropKey, device, out=None):
     """Synthetic propagator of `key` generated on the device (the stand-in for
-    loading a solved propagator into HBM)."""
+    loading a solved propagator into HBM), written into `out` if given."""
     Lx, Ly, Lz, Lt = key.dims
     V = Lx * Ly * Lz * Lt
+    cdt = torch.complex128 if key.dtype == "complex128" else torch.complex64
+    real = torch.float64 if key.dtype == "complex128" else torch.float32
+    if out is None:
+        out = torch.empty((V, 4, 4, 3, 3), dtype=cdt, device=device)
     g = torch.Generator(device=device)
     g.manual_seed(key.seed)
-    real = torch.float64 if key.dtype == "complex128" else torch.float32
-    p = torch.randn((V, 144, 2), dtype=real, device=device, generator=g)
-    p = torch.view_as_complex(p).reshape(Lt, -1, 144)
+    torch.randn((V, 144, 2), dtype=real, device=device, generator=g, out=torch.view_as_real(out).view(V, 144, 2))
     prof = torch.exp(-key.alpha * torch.abs(torch.arange(Lt, dtype=torch.float64, device=device) - key.t0))
-    p *= prof.to(real)[:, None, None]
-    return p.reshape(V, 4, 4, 3, 3)
+    out.view(Lt, -1, 144).mul_(prof.to(real)[:, None, None])
+    return out
 
 
 @dataclasses.dataclass
@@ -232,13 +244,19 @@ class Engine:
         self.poll_s = poll_s
         self.pool = PropagatorPool(cap_bytes, self._materialise)
         self._rr = 0
+        self._spare: Dict[int, Entry] = {}
         self.launches = {"meson_batches": 0, "meson_requests": 0, "baryon": 0, "diquark": 0}
 
     # -- pool ----------------------------------------------------------------
-    def _materialise(self, key):
+    def _materialise(self, key, recycled=None):
         torch = self.torch
         with torch.cuda.stream(self.load_stream):
-            t = device_propagator(torch, key, self.device)
+            reuse = recycled.tensor if recycled is not None else None
+            if reuse is not None and tuple(reuse.shape) == (int(np.prod(key.dims)), 4, 4, 3, 3):
+                self.load_stream.wait_event(recycled.ready)
+            else:
+                reuse = None
+            t = device_propagator(torch, key, self.device, out=reuse)
             ev = torch.cuda.Event()
             ev.record(self.load_stream)
         return Entry(t, ev)
@@ -319,7 +337,12 @@ class Engine:
         key = r.out_key()
         with torch.cuda.stream(stream):
             pi, pj = self._inputs(r, stream)
-            out = torch.empty_like(pi)
+            spare = self._spare.pop(r.rid, None)
+            if spare is not None and spare.tensor is not None and spare.tensor.shape == pi.shape:
+                stream.wait_event(spare.ready)
+                out = spare.tensor
+            else:
+                out = torch.empty_like(pi)
             (gij, gkl), = r.gammas[:1]
             with pc.abi_stream(stream):
                 pc.baryon_diquark(out, pi, pj, int(np.prod(r.dims)), int(gij), int(gkl))
@@ -384,8 +407,10 @@ class Engine:
                 queue.popleft()
                 for k in r.props:
                     self.pool.acquire(k)
-                if r.kind == "diquark":
-                    self.pool.insert(r.out_key(), None)  # reserve the output's bytes
+                if r.kind == "diquark":  # reserve the output's bytes
+                    spare = self.pool.insert(r.out_key(), None)
+                    if spare is not None:
+                        self._spare[r.rid] = spare
                 r.admitted = time.perf_counter() - t_start
                 admitted.append(r)
                 if r.kind != "meson":

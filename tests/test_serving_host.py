@@ -13,9 +13,9 @@ NB = 512 * 2304
 def pool(cap_entries):
     made = []
 
-    def materialise(key):
+    def materialise(key, recycled=None):
         made.append(key)
-        return ("tensor", key)
+        return ("tensor", key, recycled)
 
     return PropagatorPool(cap_entries * NB, materialise), made
 
@@ -66,6 +66,20 @@ def test_fits_counts_resident_inputs_and_outputs():
     assert not p.fits([a, c], [out])
     p.insert(out, None)
     assert p.pinned_bytes() == 2 * NB
+
+
+def test_eviction_hands_back_same_size_memory():
+    p, _ = pool(2)
+    a, b, c = (PropKey(SMALL, i) for i in range(3))
+    va = p.acquire(a)
+    p.release(a)
+    p.acquire(b)
+    vc = p.acquire(c)  # evicts a, whose memory has exactly c's size
+    assert vc[2] is va
+    big = OutKey(7, (8, 4, 4, 8))  # twice the size: nothing of that size to reuse
+    p.release(b)
+    p.release(c)
+    assert p.insert(big, None) is None
 
 
 def test_entry_larger_than_cap_is_rejected():
