@@ -219,7 +219,7 @@ GRID_LATTICES = [(16, 16, 16, 32)] * 8 + [(24, 24, 24, 48)] * 8 + [(32, 32, 32, 
 MOM7 = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
 
 
-def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle):
+def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle, fixed_tuning=None, precisions=("fp64", "fp32")):
     """Batched meson all-to-all: 24 uneven requests (8 each on 16^3x32,
     24^3x48, 32^3x64), each a pair of FP64 propagators (exp(-0.25|t - t0|),
     t0 drawn per request from a seeded RNG), full 16x16 Gamma x Gamma' grid at
@@ -239,12 +239,17 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle):
            "sites": sites}
     outs = {}
     for name, batch, bps in (("fp64", reqs, 2 * 2304), ("fp32", None, 2 * 1152)):
+        if name not in precisions:
+            continue
         if batch is None:
             batch = [(a.to(torch.complex64), b.to(torch.complex64), d, o) for a, b, d, o in reqs]
             del reqs[:]
         torch.cuda.synchronize()
         t_tune = time.perf_counter()
-        best, table = autotune.tune_meson_grid(batch, MOM7, reps=2)
+        if fixed_tuning is None:
+            best, table = autotune.tune_meson_grid(batch, MOM7, reps=2)
+        else:
+            best, table = tuple(fixed_tuning), {}
         t_tune = time.perf_counter() - t_tune
         ws = torch.empty(pc.meson_grid_workspace_bytes(batch, 7, best), dtype=torch.uint8, device=dev)
         out = [torch.empty((16, 16, 7, d[3]), dtype=torch.complex128, device=dev) for _, _, d, _ in batch]
@@ -280,6 +285,8 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle):
             num = np.abs(outs[name][0] - want).max(axis=2)
             res[name]["gpu_vs_oracle_rel_err_first_request"] = float((num / np.abs(want).max(axis=2)).max())
         del batch, out, ws, graph
+    if len(outs) < 2:
+        return res
     err = 0.0
     for got, want in zip(outs["fp32"], outs["fp64"]):
         num = np.abs(got - want).max(axis=(0, 1, 2))
@@ -314,6 +321,9 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the 32^3x64 FP32 precision study")
     ap.add_argument("--no-grid", action="store_true", help="skip the batched meson grid leg")
     ap.add_argument("--only-grid", action="store_true", help="run only the batched meson grid leg")
+    ap.add_argument("--grid-tuning", default=None,
+                    help="sites_per_item,tile_sites,blocks_per_cu for the grid leg instead of autotuning")
+    ap.add_argument("--grid-precision", default="fp64,fp32", help="precisions of the grid leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sweep")
     ap.add_argument("--only-serving", action="store_true", help="run only the serving-loop sweep")
     ap.add_argument("--serving-rates", default="10,25,50,100,200", help="offered request rates (req/s)")
@@ -339,8 +349,10 @@ def main():
         return
 
     if args.only_grid:
+        tuning = [int(v) for v in args.grid_tuning.split(",")] if args.grid_tuning else None
         print(json.dumps({"meson_grid_batch": meson_grid_batch(torch, pc, dev, args.steps, args.warmup, hip, timer,
-                                                                not args.no_cpu)}))
+                                                                not args.no_cpu, tuning,
+                                                                args.grid_precision.split(","))}))
         return
 
     Lx, Ly, Lz, Lt = DIMS

@@ -72,7 +72,7 @@ __device__ __forceinline__ void diquark_tensor(C (&D)[9], const C *__restrict__ 
 template <typename C> struct KlLane {
   int kx, ky;
   C ux, vx, uy, vy;
-  C w0, w2; // weights of D in the Z = m tails (IK_JL_NM, IL_JK_NM); 0 off-pattern
+  bool on0, on2; // lanes whose D enters the Z = m tails (IK_JL_NM, IL_JK_NM)
   __device__ __forceinline__ KlLane(const SparseSpin &gkl, const SparseSpin &gklT, int q)
   {
     const int x = q >> 2, y = q & 3;
@@ -82,8 +82,13 @@ template <typename C> struct KlLane {
     vx = sparse_val<C>(gklT, x);
     uy = sparse_val<C>(gkl, y);
     vy = sparse_val<C>(gklT, y);
-    w0 = (x == ky) ? vy : czero<C>(); // D[k(l)][l] with l = y
-    w2 = (y == kx) ? vx : czero<C>(); // D[l][k(l)] with l = x
+    on0 = x == ky; // D[k(l)][l] with l = y
+    on2 = y == kx; // D[l][k(l)] with l = x
+  }
+  // weight of this lane's D in the Z = m tail of type T (0 off-pattern)
+  template <int T> __device__ __forceinline__ C wz() const
+  {
+    return (T == 0) ? (on0 ? vy : czero<C>()) : (on2 ? vx : czero<C>());
   }
 };
 
@@ -101,7 +106,7 @@ __device__ __forceinline__ C baryon_tail(const C (&D)[9], const C *__restrict__ 
   constexpr int Z = type_slot(T, 2);
   C a[9];
   if constexpr (Z == LM && !LDS_SUM) {
-    const C w = (T == 0) ? kl.w0 : kl.w2;
+    const C w = kl.template wz<T>();
     C x1[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) x1[e] = group16_allreduce(cmul(w, D[e]));
@@ -111,7 +116,7 @@ __device__ __forceinline__ C baryon_tail(const C (&D)[9], const C *__restrict__ 
     // R[n][m] = sum_ad P_n[n][m] X1, X1 = sum_l Gkl[k,l] D[k][l] (T = 0) or D[l][k] (T = 2):
     // the four lanes holding a D[k(l)][l] (or D[l][k(l)]) post it weighted to
     // LDS slot l, nine lanes sum the slots entry-wise, every lane reads X1.
-    const C w = (T == 0) ? kl.w0 : kl.w2;
+    const C w = kl.template wz<T>();
     const int slot = (T == 0) ? (q & 3) : (q >> 2);
     __syncthreads();
     if (w.x != 0 || w.y != 0)
@@ -221,58 +226,123 @@ void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream)
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void site_coords(size_t s, const int (&dims)[4], int order, int &x, int &y, int &z, int &t)
+__device__ __forceinline__ int nonneg_mod(int a, int L)
 {
-  if (order == 0) {
-    x = (int)(s % dims[0]);
-    s /= dims[0];
-    y = (int)(s % dims[1]);
-    s /= dims[1];
-    z = (int)(s % dims[2]);
-    t = (int)(s / dims[2]);
-  } else {
-    const size_t half = (size_t)dims[0] * dims[1] * dims[2] * dims[3] / 2;
-    const int parity = (int)(s / half);
-    size_t r = s - (size_t)parity * half;
-    const int lxh = dims[0] >> 1;
-    const int xh = (int)(r % lxh);
-    r /= lxh;
-    y = (int)(r % dims[1]);
-    r /= dims[1];
-    z = (int)(r % dims[2]);
-    t = (int)(r / dims[2]);
-    x = 2 * xh + ((parity + y + z + t) & 1);
-  }
+  const int r = a % L;
+  return r < 0 ? r + L : r;
 }
 
-// Phase exp(-2 pi i p.x / L) of momentum p at spatial site (x, y, z), from
-// per-direction tables held in LDS (tab = [Lx | Ly | Lz] entries).
-__device__ __forceinline__ double2 momentum_phase(const ProjectArgs &pa, const double2 *tab, int p, int x, int y,
-                                                  int z)
+// Phases exp(-2 pi i p.x / L) of this lane's MFMA rows (momenta p = tt*16 + q)
+// at this lane's site, walked along the work item without integer division:
+// the lane's site advances by SITES_PER_WAVE per step, and the table indices
+// k_mu = (p_mu x_mu) mod L_mu move by fixed increments (a wrap of x_mu by L_mu
+// leaves k_mu unchanged).  A slice is [z][y][xr] with xr = x (lexicographic)
+// or x / 2 (even-odd: x = 2 xr + ((parity + y + z + t) & 1)).
+// Remainder of 0 <= a < 2^24 by 0 < L, from a float reciprocal: the quotient
+// estimate is off by at most one, which the two selects correct.
+__device__ __forceinline__ int mod_small(int a, int L, float inv_l)
 {
-  if (p >= pa.np) return make_double2(0.0, 0.0);
-  int kx = (pa.mom[p][0] * x) % pa.dims[0];
-  int ky = (pa.mom[p][1] * y) % pa.dims[1];
-  int kz = (pa.mom[p][2] * z) % pa.dims[2];
-  kx += kx < 0 ? pa.dims[0] : 0;
-  ky += ky < 0 ? pa.dims[1] : 0;
-  kz += kz < 0 ? pa.dims[2] : 0;
-  return cmul(cmul(tab[kx], tab[pa.dims[0] + ky]), tab[pa.dims[0] + pa.dims[1] + kz]);
+  int r = a - L * (int)((float)a * inv_l);
+  r += r < 0 ? L : 0;
+  r -= r >= L ? L : 0;
+  return r;
 }
+
+template <int NT> struct PhaseWalk {
+  static_assert(MAX_PHASE_TAB < 512, "table indices are packed in 9-bit fields");
+  int xr, y, z;  // lane coordinates
+  int Lr, pbase; // row length of a slice, parity + t (even-odd)
+  float inv_lx;
+  // per momentum tile, three 9-bit fields per register (every L_mu < 512):
+  //   k = kx | ky << 9 | kz << 18, kx = (p_x x) mod Lx (or (2 p_x xr) mod Lx
+  //   for even-odd), ky = (p_y y) mod Ly, kz = (p_z z) mod Lz;
+  //   m = p_x | p_y << 9 | p_z << 18 (p_mu mod L_mu; 0 for rows past np)
+  int k[NT], m[NT];
+
+  __device__ __forceinline__ void init(const ProjectArgs &pa, size_t first, int g, int q)
+  {
+    const size_t slice = first / pa.slice_sites;
+    const unsigned r0 = (unsigned)(first - slice * pa.slice_sites) + (unsigned)g;
+    const int lt = pa.dims[3];
+    pbase = (int)(slice / lt) + (int)(slice % lt);
+    Lr = pa.order ? pa.dims[0] >> 1 : pa.dims[0];
+    inv_lx = 1.0f / (float)pa.dims[0];
+    xr = (int)(r0 % (unsigned)Lr);
+    const unsigned rest = r0 / (unsigned)Lr;
+    y = (int)(rest % (unsigned)pa.dims[1]);
+    z = (int)(rest / (unsigned)pa.dims[1]);
+    const int xs = pa.order ? 2 : 1;
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt) {
+      const int p = tt * 16 + q;
+      const bool live = p < pa.np;
+      const int mx = live ? nonneg_mod(pa.mom[p][0], pa.dims[0]) : 0;
+      const int my = live ? nonneg_mod(pa.mom[p][1], pa.dims[1]) : 0;
+      const int mz = live ? nonneg_mod(pa.mom[p][2], pa.dims[2]) : 0;
+      const int kx = (xs * mx * xr) % pa.dims[0];
+      const int ky = (my * y) % pa.dims[1];
+      const int kz = (mz * (z % pa.dims[2])) % pa.dims[2];
+      k[tt] = kx | ky << 9 | kz << 18;
+      m[tt] = mx | my << 9 | mz << 18;
+    }
+  }
+
+  __device__ __forceinline__ double2 phase(const ProjectArgs &pa, const double2 *tab, int tt) const
+  {
+    int kx = k[tt] & 511;
+    if (pa.order && ((pbase + y + z) & 1)) {
+      kx += m[tt] & 511;
+      kx -= kx >= pa.dims[0] ? pa.dims[0] : 0;
+    }
+    return cmul(cmul(tab[kx], tab[pa.dims[0] + ((k[tt] >> 9) & 511)]),
+                tab[pa.dims[0] + pa.dims[1] + (k[tt] >> 18)]);
+  }
+
+  __device__ __forceinline__ void advance(const ProjectArgs &pa)
+  {
+    const int xstep = (pa.order ? 2 : 1) * SITES_PER_WAVE;
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt) {
+      const int kx = mod_small((k[tt] & 511) + xstep * (m[tt] & 511), pa.dims[0], inv_lx);
+      k[tt] = (k[tt] & ~511) | kx;
+    }
+    xr += SITES_PER_WAVE;
+    while (xr >= Lr) {
+      xr -= Lr;
+      ++y;
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt) {
+        int ky = ((k[tt] >> 9) & 511) + ((m[tt] >> 9) & 511);
+        ky -= ky >= pa.dims[1] ? pa.dims[1] : 0;
+        k[tt] = (k[tt] & ~(511 << 9)) | ky << 9;
+      }
+      if (y == pa.dims[1]) {
+        y = 0;
+        ++z;
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt) {
+          int kz = (k[tt] >> 18) + (m[tt] >> 18);
+          kz -= kz >= pa.dims[2] ? pa.dims[2] : 0;
+          k[tt] = (k[tt] & ((1 << 18) - 1)) | kz << 18;
+        }
+      }
+    }
+  }
+};
 
 template <int NT>
 __device__ __forceinline__ void project_accumulate(dbl4 (&acc_re)[NT], dbl4 (&acc_im)[NT], const ProjectArgs &pa,
-                                                   const double2 *tab, double2 R, size_t site, int q)
+                                                   const double2 *tab, double2 R, const PhaseWalk<NT> &walk)
 {
-  int x, y, z, t;
-  site_coords(site, pa.dims, pa.order, x, y, z, t);
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
-    const double2 ph = momentum_phase(pa, tab, tt * 16 + q, x, y, z);
-    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.x, R.x, acc_re[tt], 0, 0, 0);
-    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ph.y, R.y, acc_re[tt], 0, 0, 0);
-    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.x, R.y, acc_im[tt], 0, 0, 0);
-    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.y, R.x, acc_im[tt], 0, 0, 0);
+    const double2 ph = walk.phase(pa, tab, tt); // rows past np have p_mu = 0 but R-weighted by a zero below
+    const bool live = tt * 16 + (int)(threadIdx.x & 15) < pa.np;
+    const double pr = live ? ph.x : 0.0, pi = live ? ph.y : 0.0;
+    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr, R.x, acc_re[tt], 0, 0, 0);
+    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-pi, R.y, acc_re[tt], 0, 0, 0);
+    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr, R.y, acc_im[tt], 0, 0, 0);
+    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi, R.x, acc_im[tt], 0, 0, 0);
   }
 }
 
@@ -361,6 +431,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT
   item_range(pa, item, first, last);
   load_phase_tab(tab, pa, lane);
   const KlLane<C> kl(a.gkl, a.gklT, q);
+  PhaseWalk<NT> walk;
+  walk.init(pa, first, g, q);
 
   dbl4 acc_re[NT], acc_im[NT];
 #pragma unroll
@@ -400,8 +472,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT
       R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)r.x, (double)r.y));
     }
     if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
-    const size_t site = (g < nsites) ? s + g : s; // padded lanes carry R = 0
-    project_accumulate<NT>(acc_re, acc_im, pa, tab, R, site, q);
+    project_accumulate<NT>(acc_re, acc_im, pa, tab, R, walk); // padded lanes carry R = 0
+    walk.advance(pa);
     __syncthreads();
   }
   project_store<NT>(acc_re, acc_im, pa, item, lane);
