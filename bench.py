@@ -92,6 +92,19 @@ def make_propagators(torch, dims, seeds, alpha, device, t0=0):
     return out
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the bench workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json,
+    fetch scaled by 2.0 and write by 1.0 against a plain copy kernel)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"][kernel]
+        return k["traffic_bytes"], "profiles/traffic.json: " + k["workload"]
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def time_steps(torch, fn, steps, warmup, hip=None, timer=None):
     """Run warmup + timed steps; returns (wall ms/step, kernel ms/step or None)."""
     for _ in range(warmup):
@@ -277,6 +290,8 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle, fi
             "hbm_frac": gbs / HBM_PEAK_GBS,
             "mfma_TFLOPs": 16384.0 * sites / (kern * 1e-3) / 1e12,
         }
+        if name == "fp64":
+            res[name]["traffic"], res[name]["traffic_source"] = measured_traffic("meson_grid_kernel<double,4>")
         outs[name] = [o.cpu().numpy() for o in out]
         if name == "fp64" and check_oracle:
             import oracle
@@ -417,6 +432,7 @@ def main():
         cpu = cpu_baseline(torch, (u, d, u2), gpu_sample)
 
     achieved = BYTES_PER_SITE_FP64 * V / (kern64 * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic("baryon_project_kernel<2,double>")
     result = {
         "metric": "baryon 2pt sites/s & % HBM roofline, FP32 vs FP64; p50/p99 request latency",
         "value": V / (wall64 * 1e-3),
@@ -445,7 +461,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "baryon_project_kernel<2,double>",
             "kernel_ms": kern64,
             "bytes_per_site": BYTES_PER_SITE_FP64,

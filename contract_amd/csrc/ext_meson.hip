@@ -200,7 +200,7 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
       O.block0 = blocks;
       items += p.items;
       blocks += q.dims[3];
-      tab = std::max(tab, q.dims[0] + q.dims[1] + q.dims[2]);
+      tab = std::max(tab, q.dims[0] + q.dims[1] + q.dims[2] + 3);
     }
     a.total_items = (unsigned)items;
     const size_t grid = std::min(items, (size_t)device_cu_count() * g.bpc);

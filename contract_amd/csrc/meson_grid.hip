@@ -101,15 +101,17 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
 
     __syncthreads(); // the previous item's readers of tab are done
     // per-momentum phase tables exp(-2 pi i p_mu k / L_mu), k < L_mu:
-    // [np][Lx] | [np][Ly] | [np][Lz], so a lane's phase needs no division
+    // [np][Lx + 1] | [np][Ly + 1] | [np][Lz + 1], so a lane's phase needs no
+    // division; the padding entry keeps rows of power-of-two length from
+    // starting on the same LDS bank (the eight rows a wave reads at once)
     const int Lx = R.dims[0], Ly = R.dims[1], Lz = R.dims[2];
-    const int offy = a.np * Lx, offz = a.np * (Lx + Ly);
-    const int ntab = a.np * (Lx + Ly + Lz);
+    const int offy = a.np * (Lx + 1), offz = a.np * (Lx + Ly + 2);
+    const int ntab = a.np * (Lx + Ly + Lz + 3);
     for (int e = tid; e < ntab; e += GRID_THREADS) {
       const int mu = e < offy ? 0 : (e < offz ? 1 : 2);
       const int L = R.dims[mu];
       const int rel = e - (mu == 0 ? 0 : (mu == 1 ? offy : offz));
-      const int p = rel / L, k = rel - p * L;
+      const int p = rel / (L + 1), k = rel - p * (L + 1);
       const int m = nonneg_mod(a.mom[p][mu], L);
       double sn, cs;
       sincospi(2.0 * (double)((m * k) % L) / (double)L, &sn, &cs);
@@ -119,7 +121,7 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
     const int pr = q >> 1;
     const bool live = pr < a.np;
     const int prow = live ? pr : 0;
-    const double2 *tx = tab + prow * Lx, *ty = tab + offy + prow * Ly, *tz = tab + offz + prow * Lz;
+    const double2 *tx = tab + prow * (Lx + 1), *ty = tab + offy + prow * (Ly + 1), *tz = tab + offz + prow * (Lz + 1);
 
     // coordinates of the step's first site, advanced incrementally (uniform):
     // a slice is [z][y][xr] with xr = x (lexicographic) or x / 2 (even-odd,

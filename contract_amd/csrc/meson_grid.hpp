@@ -50,7 +50,7 @@ struct MesonGridFinish {
 };
 
 // tile_sites: sites staged per step (4 or 8); blocks: persistent grid size;
-// tab_entries: np x max(Lx + Ly + Lz), the per-momentum phase tables in LDS.
+// tab_entries: np x max(Lx + Ly + Lz + 3), the per-momentum phase tables in LDS.
 void launch_meson_grid(const MesonGridArgs &a, bool fp32, int tile_sites, size_t blocks, size_t tab_entries,
                        hipStream_t stream);
 void launch_meson_grid_finish(const MesonGridFinish &f, int nblocks, hipStream_t stream);
