@@ -473,6 +473,7 @@ def main():
             "kernel_ms": kern32,
             "achieved_GBs": BYTES_PER_SITE_FP32 * V / (kern32 * 1e-3) / 1e9,
             "speedup_vs_fp64": wall64 / wall32,
+            "kernel_speedup_vs_fp64": kern64 / kern32,
             "per_timeslice_rel_err_vs_fp64": err32,
         },
         "abi_baryon_two_point": {

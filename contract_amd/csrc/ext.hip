@@ -3,6 +3,7 @@
 
 #include <contract_ext.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
 #include <vector>
@@ -37,9 +38,12 @@ void set_last_error(const char *fmt, ...)
 
 int device_cu_count()
 {
+  static std::atomic<int> cache[64]; // per device, 0 = not yet queried
   int dev = 0, n = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (dev >= 0 && dev < 64 && (n = cache[dev].load(std::memory_order_relaxed)) > 0) return n;
   if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+  if (dev >= 0 && dev < 64) cache[dev].store(n, std::memory_order_relaxed);
   return n;
 }
 

@@ -204,14 +204,30 @@ def baryonSequentialTwoPoint(propag_i, propag_j, propag_n, contract_type, sequen
     return (propag_i, propag_j, propag_n)[int(sequential_type)]
 
 
+_GAMMA_CACHE: dict = {}
+_PROJ_CACHE: dict = {}
+
+
 def _projectors(insertions) -> np.ndarray:
+    """[n][4][4] complex128 projectors; gamma-index lists are cached so the
+    per-call host cost stays small next to the kernel."""
+    key = tuple(insertions) if all(isinstance(w, (int, np.integer)) for w in insertions) else None
+    if key is not None and key in _PROJ_CACHE:
+        return _PROJ_CACHE[key]
     mats = []
     for w in insertions:
         if isinstance(w, (int, np.integer)):
-            mats.append(gamma_matrix(int(w)))
+            g = int(w)
+            if g not in _GAMMA_CACHE:
+                _GAMMA_CACHE[g] = gamma_matrix(g)
+            mats.append(_GAMMA_CACHE[g])
         else:
             mats.append(np.asarray(w, dtype=np.complex128).reshape(4, 4))
-    return np.ascontiguousarray(np.stack(mats))
+    out = np.ascontiguousarray(np.stack(mats))
+    if key is not None:
+        out.setflags(write=False)
+        _PROJ_CACHE[key] = out
+    return out
 
 
 def project_workspace_bytes(dims: Sequence[int], nmom: int, site_order: int = LEXICOGRAPHIC) -> int:
