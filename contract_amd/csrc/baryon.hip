@@ -66,6 +66,39 @@ __device__ __forceinline__ void diquark_tensor(C (&D)[9], const C *__restrict__ 
   }
 }
 
+// Rows of a gamma matrix known at compile time: row i has its one nonzero in
+// column col[i], equal to i^rot[i].  G(5) = Cg5 in the DeGrand-Rossi basis
+// (gamma_1 gamma_3), the diquark projector of the nucleon, is instantiated;
+// any other Gij takes the runtime path (the reference instantiates its kernels
+// per gamma index in the same way, lib/baryon/baryon.cu:22-67).
+template <int G> struct GammaRows;
+template <> struct GammaRows<5> {
+  static constexpr int col[4] = {1, 0, 3, 2};
+  static constexpr int rot[4] = {2, 0, 2, 0}; // -1, +1, -1, +1
+};
+
+// diquark_tensor for a compile-time Gij: the weights become operand signs.
+template <int GIJ, int I, typename C>
+__device__ __forceinline__ void diquark_row(C (&D)[9], const C *__restrict__ si, const C *__restrict__ sj, int x, int y)
+{
+  C b[9];
+  load_block(b, sj, GammaRows<GIJ>::col[I], y);
+  cof_acc_stream_rot<GammaRows<GIJ>::rot[I]>(D, si + (I * 4 + x) * 9, b);
+}
+
+template <int GIJ, typename C>
+__device__ __forceinline__ void diquark_tensor_static(C (&D)[9], const C *__restrict__ si, const C *__restrict__ sj,
+                                                      int q)
+{
+  const int x = q >> 2, y = q & 3;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) D[e] = czero<C>();
+  diquark_row<GIJ, 0>(D, si, sj, x, y);
+  diquark_row<GIJ, 1>(D, si, sj, x, y);
+  diquark_row<GIJ, 2>(D, si, sj, x, y);
+  diquark_row<GIJ, 3>(D, si, sj, x, y);
+}
+
 // Per-lane constants of the Gkl tails, computed once per kernel.  For lane
 // (x, y): kx = col(x), ky = col(y) of Gkl (its pattern is symmetric), and the
 // values Gkl[x][kx] (u), Gkl[kx][x] (v), likewise for y.
@@ -387,11 +420,11 @@ __device__ __forceinline__ void item_range(const ProjectArgs &pa, size_t item, s
   first = first < s1 ? first : s1;
 }
 
-// FP32 storage: rescale every staged site of every propagator by an exact
-// power of two so its largest entry lies in [1, 2).  The contraction is
-// trilinear, so the FP32 result times 2^(e_i + e_j + e_n) is the true value;
-// the product of three steeply decaying propagators (~1e-66 at the far end of
-// a 64-timeslice lattice) therefore never underflows FP32.
+// FP32 storage: rescale a staged site of a propagator by an exact power of
+// two so its largest entry lies in [1, 2).  The contraction is trilinear, so
+// rescaling P_i and P_j makes the FP32 result times 2^(e_i + e_j) the true
+// value; the product of three steeply decaying propagators (~1e-66 at the far
+// end of a 64-timeslice lattice) therefore never underflows FP32.
 __device__ __forceinline__ int normalise_site(float2 *site_lds, int q)
 {
   float2 *p = site_lds + q * 9;
@@ -415,7 +448,7 @@ __device__ __forceinline__ int normalise_site(float2 *site_lds, int q)
 #define CONTRACT_WPE32 2
 #endif
 
-template <int NT, typename Real>
+template <int NT, typename Real, int GIJ = -1>
 __global__ __launch_bounds__(WAVE)
 __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT_WPE32))) void baryon_project_kernel(
   BaryonProjectArgs a, ProjectArgs pa)
@@ -461,11 +494,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT
     C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[2] + g * SL;
     int ex = 0;
     if constexpr (sizeof(Real) == 4) {
-      ex = normalise_site(si, q) + normalise_site(sj, q) + normalise_site(sn, q);
+      // P_i and P_j only: D of the rescaled blocks is O(1), so the tails'
+      // products with P_n are as large as P_n's own entries and cannot
+      // underflow unless a stored entry already is subnormal
+      ex = normalise_site(si, q) + normalise_site(sj, q);
       __syncthreads();
     }
     C D[9];
-    diquark_tensor(D, si, sj, q, a.gij);
+    if constexpr (GIJ >= 0)
+      diquark_tensor_static<GIJ>(D, si, sj, q);
+    else
+      diquark_tensor(D, si, sj, q, a.gij);
     double2 R = make_double2(0.0, 0.0);
     for (int t = 0; t < a.nterm; ++t) {
       const C r = baryon_tail_rt<C>(a.types[t], D, sn, si, lane, kl);
@@ -479,18 +518,27 @@ __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT
   project_store<NT>(acc_re, acc_im, pa, item, lane);
 }
 
-template <typename Real>
-static void launch_project_t(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+template <typename Real, int GIJ>
+static void launch_project_g(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
 {
   const int nt = (pa.np + 15) / 16;
   const dim3 grid((unsigned)items), block(WAVE);
   const size_t tab_bytes = sizeof(double2) * (pa.dims[0] + pa.dims[1] + pa.dims[2]);
   switch (nt) {
-  case 1: hipLaunchKernelGGL((baryon_project_kernel<1, Real>), grid, block, tab_bytes, stream, a, pa); break;
-  case 2: hipLaunchKernelGGL((baryon_project_kernel<2, Real>), grid, block, tab_bytes, stream, a, pa); break;
-  case 3: hipLaunchKernelGGL((baryon_project_kernel<3, Real>), grid, block, tab_bytes, stream, a, pa); break;
-  default: hipLaunchKernelGGL((baryon_project_kernel<4, Real>), grid, block, tab_bytes, stream, a, pa); break;
+  case 1: hipLaunchKernelGGL((baryon_project_kernel<1, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
+  case 2: hipLaunchKernelGGL((baryon_project_kernel<2, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
+  case 3: hipLaunchKernelGGL((baryon_project_kernel<3, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
+  default: hipLaunchKernelGGL((baryon_project_kernel<4, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
   }
+}
+
+template <typename Real>
+static void launch_project_t(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+{
+  if (a.gamma_ij == 5)
+    launch_project_g<Real, 5>(a, pa, items, stream);
+  else
+    launch_project_g<Real, -1>(a, pa, items, stream);
 }
 
 void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)

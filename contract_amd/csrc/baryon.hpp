@@ -46,6 +46,7 @@ struct ProjectArgs {
 
 struct BaryonProjectArgs {
   const void *propag_i, *propag_j, *propag_n;
+  int gamma_ij; // selects a compile-time Gij form where one is instantiated
   SparseSpin gij, gkl, gklT;
   int nterm;
   int types[MAX_TERMS];

@@ -91,6 +91,26 @@ template <typename C> __device__ __forceinline__ C cfma_conj(C acc, C a, C b)
   acc.y = fma(-a.x, b.y, acc.y);
   return acc;
 }
+// acc + i^R * a * b for a compile-time unit i^R (R = 0..3): the unit only
+// moves and negates FMA operands
+template <int R, typename C> __device__ __forceinline__ C cfma_rot(C acc, C a, C b)
+{
+  constexpr int r = R & 3;
+  if constexpr (r == 0) return cfma(acc, a, b);
+  if constexpr (r == 2) return cfms(acc, a, b);
+  if constexpr (r == 1) { // acc.x -= Im(ab), acc.y += Re(ab)
+    acc.x = fma(-a.x, b.y, acc.x);
+    acc.x = fma(-a.y, b.x, acc.x);
+    acc.y = fma(a.x, b.x, acc.y);
+    acc.y = fma(-a.y, b.y, acc.y);
+  } else { // r == 3: acc.x += Im(ab), acc.y -= Re(ab)
+    acc.x = fma(a.x, b.y, acc.x);
+    acc.x = fma(a.y, b.x, acc.x);
+    acc.y = fma(-a.x, b.x, acc.y);
+    acc.y = fma(a.y, b.y, acc.y);
+  }
+  return acc;
+}
 template <typename C> __device__ __forceinline__ C cconj(C a) { return cmake<C>(a.x, -a.y); }
 template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) { return cmake<C>(a.x * s, a.y * s); }
 
@@ -321,6 +341,25 @@ __device__ __forceinline__ void cof_acc_stream(C (&acc)[9], const C *__restrict_
       acc[a1 * 3 + d2] = cfms(acc[a1 * 3 + d2], y, Z[a2 * 3 + d1]);
       acc[a2 * 3 + d1] = cfms(acc[a2 * 3 + d1], y, Z[a1 * 3 + d2]);
       acc[a2 * 3 + d2] = cfma(acc[a2 * 3 + d2], y, Z[a1 * 3 + d1]);
+    }
+  }
+}
+
+// acc += i^R cof(Y, Z), the unit folded into the FMAs (see cof_acc_stream)
+template <int R, typename C>
+__device__ __forceinline__ void cof_acc_stream_rot(C (&acc)[9], const C *__restrict__ Yp, const C (&Z)[9])
+{
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const int a1 = (b + 1) % 3, a2 = (b + 2) % 3;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      const int d1 = (e + 1) % 3, d2 = (e + 2) % 3;
+      const C y = Yp[b * 3 + e];
+      acc[a1 * 3 + d1] = cfma_rot<R>(acc[a1 * 3 + d1], y, Z[a2 * 3 + d2]);
+      acc[a1 * 3 + d2] = cfma_rot<R + 2>(acc[a1 * 3 + d2], y, Z[a2 * 3 + d1]);
+      acc[a2 * 3 + d1] = cfma_rot<R + 2>(acc[a2 * 3 + d1], y, Z[a1 * 3 + d2]);
+      acc[a2 * 3 + d2] = cfma_rot<R>(acc[a2 * 3 + d2], y, Z[a1 * 3 + d1]);
     }
   }
 }

@@ -265,6 +265,7 @@ int contract_baryon_twopoint_project(void *out, const void *propag_i, const void
   a.propag_i = propag_i;
   a.propag_j = propag_j;
   a.propag_n = propag_n;
+  a.gamma_ij = gamma_ij;
   a.gij = sparse_of(gamma_matrix(gamma_ij));
   a.gkl = sparse_of(gamma_matrix(gamma_kl));
   a.gklT = sparse_of(mat_transpose(gamma_matrix(gamma_kl)));

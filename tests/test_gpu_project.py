@@ -68,6 +68,15 @@ def test_every_type_and_general_projector(dev, t):
     assert per_timeslice_error(got, want) < 1e-12
 
 
+@pytest.mark.parametrize("gij", range(16))
+@pytest.mark.parametrize("t", [0, 5])
+def test_every_diquark_gamma(dev, gij, t):
+    """Gij = G(5) (Cg5) runs a compile-time form of the diquark, every other
+    Gij the runtime one; both against the oracle."""
+    got, want = run_case(dev, (2, 2, 4, 3), 0, [(t, 1.0)], [0, 5, 15], MOM7[:3], gij=gij, gkl=(gij + 3) % 16)
+    assert per_timeslice_error(got, want) < 1e-12
+
+
 def test_ragged_slices(dev):
     # 3*3*3 = 27 sites per timeslice: work items end mid-way through a 4-site step
     got, want = run_case(dev, (3, 3, 3, 5), 0, PROTON, [0, 8], MOM7)
