@@ -114,6 +114,7 @@ static void baryon_site(void *correl, void *propag_i, void *propag_j, void *prop
   a.propag_n = propag_n;
   a.correl = correl;
   a.volume = volume;
+  a.gamma_ij = gamma_ij;
   a.gij = sp(gamma_ij);
   a.gkl = sp(gamma_kl);
   a.gklT = spT(gamma_kl);

@@ -208,7 +208,7 @@ __device__ __forceinline__ C baryon_tail_rt(int type, const C (&D)[9], const C *
 // baryon_two_point passes W = G(gamma_mn), baryon_general_two_point passes
 // W = project_mn (include/baryon_general.cuh:62-67).
 // ---------------------------------------------------------------------------
-template <int T>
+template <int T, int GIJ>
 __global__ __launch_bounds__(WAVE) void baryon_site_kernel(BaryonSiteArgs a)
 {
   using C = double2;
@@ -221,26 +221,39 @@ __global__ __launch_bounds__(WAVE) void baryon_site_kernel(BaryonSiteArgs a)
   const C *P[3] = {(const C *)a.propag_i, (const C *)a.propag_j, (const C *)a.propag_n};
   persistent_tiles<3>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
     C D[9];
-    diquark_tensor(D, lds + g * SL, lds + (SITES_PER_WAVE + g) * SL, q, a.gij);
+    if constexpr (GIJ >= 0)
+      diquark_tensor_static<GIJ>(D, lds + g * SL, lds + (SITES_PER_WAVE + g) * SL, q);
+    else
+      diquark_tensor(D, lds + g * SL, lds + (SITES_PER_WAVE + g) * SL, q, a.gij);
     const C R = baryon_tail<T, C, false>(D, lds + (2 * SITES_PER_WAVE + g) * SL, lds + g * SL, lane, kl);
     const C c = group16_allreduce(cmul(w, R));
     if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
   });
 }
 
+template <int GIJ>
+static void launch_site_g(int type, const BaryonSiteArgs &a, dim3 grid, hipStream_t stream)
+{
+  const dim3 block(WAVE);
+  switch (type) {
+  case 0: hipLaunchKernelGGL((baryon_site_kernel<0, GIJ>), grid, block, 0, stream, a); break;
+  case 1: hipLaunchKernelGGL((baryon_site_kernel<1, GIJ>), grid, block, 0, stream, a); break;
+  case 2: hipLaunchKernelGGL((baryon_site_kernel<2, GIJ>), grid, block, 0, stream, a); break;
+  case 3: hipLaunchKernelGGL((baryon_site_kernel<3, GIJ>), grid, block, 0, stream, a); break;
+  case 4: hipLaunchKernelGGL((baryon_site_kernel<4, GIJ>), grid, block, 0, stream, a); break;
+  default: hipLaunchKernelGGL((baryon_site_kernel<5, GIJ>), grid, block, 0, stream, a); break;
+  }
+}
+
 void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream)
 {
   const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
   if (ntiles == 0) return;
-  const dim3 grid((unsigned)persistent_grid(ntiles, 5)), block(WAVE);
-  switch (type) {
-  case 0: hipLaunchKernelGGL(baryon_site_kernel<0>, grid, block, 0, stream, a); break;
-  case 1: hipLaunchKernelGGL(baryon_site_kernel<1>, grid, block, 0, stream, a); break;
-  case 2: hipLaunchKernelGGL(baryon_site_kernel<2>, grid, block, 0, stream, a); break;
-  case 3: hipLaunchKernelGGL(baryon_site_kernel<3>, grid, block, 0, stream, a); break;
-  case 4: hipLaunchKernelGGL(baryon_site_kernel<4>, grid, block, 0, stream, a); break;
-  default: hipLaunchKernelGGL(baryon_site_kernel<5>, grid, block, 0, stream, a); break;
-  }
+  const dim3 grid((unsigned)persistent_grid(ntiles, 5));
+  if (a.gamma_ij == 5)
+    launch_site_g<5>(type, a, grid, stream);
+  else
+    launch_site_g<-1>(type, a, grid, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -367,15 +380,25 @@ template <int NT>
 __device__ __forceinline__ void project_accumulate(dbl4 (&acc_re)[NT], dbl4 (&acc_im)[NT], const ProjectArgs &pa,
                                                    const double2 *tab, double2 R, const PhaseWalk<NT> &walk)
 {
+  double pr[NT], pi[NT];
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
-    const double2 ph = walk.phase(pa, tab, tt); // rows past np have p_mu = 0 but R-weighted by a zero below
-    const bool live = tt * 16 + (int)(threadIdx.x & 15) < pa.np;
-    const double pr = live ? ph.x : 0.0, pi = live ? ph.y : 0.0;
-    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr, R.x, acc_re[tt], 0, 0, 0);
-    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-pi, R.y, acc_re[tt], 0, 0, 0);
-    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr, R.y, acc_im[tt], 0, 0, 0);
-    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi, R.x, acc_im[tt], 0, 0, 0);
+    const double2 ph = walk.phase(pa, tab, tt);
+    const bool live = tt * 16 + (int)(threadIdx.x & 15) < pa.np; // rows past np stay zero
+    pr[tt] = live ? ph.x : 0.0;
+    pi[tt] = live ? ph.y : 0.0;
+  }
+  // the two products into each accumulator are issued NT MFMAs apart, so a
+  // dependent MFMA rarely waits for the previous one to drain
+#pragma unroll
+  for (int tt = 0; tt < NT; ++tt) {
+    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.x, acc_re[tt], 0, 0, 0);
+    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.y, acc_im[tt], 0, 0, 0);
+  }
+#pragma unroll
+  for (int tt = 0; tt < NT; ++tt) {
+    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-pi[tt], R.y, acc_re[tt], 0, 0, 0);
+    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[tt], R.x, acc_im[tt], 0, 0, 0);
   }
 }
 

@@ -23,6 +23,7 @@ size_t persistent_grid(size_t ntiles, int waves_per_cu);
 // ---- baryon, drop-in per-site form -------------------------------------
 struct BaryonSiteArgs {
   const void *propag_i, *propag_j, *propag_n;
+  int gamma_ij; // selects a compile-time Gij form where one is instantiated
   void *correl;
   size_t volume;
   SparseSpin gij;  // Gij: row i -> (j, Gij[i,j])
