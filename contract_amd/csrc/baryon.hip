@@ -296,13 +296,14 @@ __device__ __forceinline__ int mod_small(int a, int L, float inv_l)
 
 template <int NT> struct PhaseWalk {
   static_assert(MAX_PHASE_TAB < 512, "table indices are packed in 9-bit fields");
-  int xr, y, z;  // lane coordinates
+  int xr, y, z;  // lane coordinates (local)
   int Lr, pbase; // row length of a slice, parity + t (even-odd)
-  float inv_lx;
-  // per momentum tile, three 9-bit fields per register (every L_mu < 512):
-  //   k = kx | ky << 9 | kz << 18, kx = (p_x x) mod Lx (or (2 p_x xr) mod Lx
-  //   for even-odd), ky = (p_y y) mod Ly, kz = (p_z z) mod Lz;
-  //   m = p_x | p_y << 9 | p_z << 18 (p_mu mod L_mu; 0 for rows past np)
+  float inv_gx, inv_gy;
+  // per momentum tile, three 9-bit fields per register (global extents G_mu < 512):
+  //   k = kx | ky << 9 | kz << 18, kx = p_x (xs xr + o_x) mod Gx with xs = 2 for
+  //   even-odd (the parity bit is added in phase()) and 1 otherwise,
+  //   ky = p_y (y + o_y) mod Gy, kz = p_z (z + o_z) mod Gz;
+  //   m = p_x | p_y << 9 | p_z << 18 (p_mu mod G_mu; 0 for rows past np)
   int k[NT], m[NT];
 
   __device__ __forceinline__ void init(const ProjectArgs &pa, size_t first, int g, int q)
@@ -312,7 +313,8 @@ template <int NT> struct PhaseWalk {
     const int lt = pa.dims[3];
     pbase = (int)(slice / lt) + (int)(slice % lt);
     Lr = pa.order ? pa.dims[0] >> 1 : pa.dims[0];
-    inv_lx = 1.0f / (float)pa.dims[0];
+    inv_gx = 1.0f / (float)pa.gdims[0];
+    inv_gy = 1.0f / (float)pa.gdims[1];
     xr = (int)(r0 % (unsigned)Lr);
     const unsigned rest = r0 / (unsigned)Lr;
     y = (int)(rest % (unsigned)pa.dims[1]);
@@ -322,12 +324,12 @@ template <int NT> struct PhaseWalk {
     for (int tt = 0; tt < NT; ++tt) {
       const int p = tt * 16 + q;
       const bool live = p < pa.np;
-      const int mx = live ? nonneg_mod(pa.mom[p][0], pa.dims[0]) : 0;
-      const int my = live ? nonneg_mod(pa.mom[p][1], pa.dims[1]) : 0;
-      const int mz = live ? nonneg_mod(pa.mom[p][2], pa.dims[2]) : 0;
-      const int kx = (xs * mx * xr) % pa.dims[0];
-      const int ky = (my * y) % pa.dims[1];
-      const int kz = (mz * (z % pa.dims[2])) % pa.dims[2];
+      const int mx = live ? nonneg_mod(pa.mom[p][0], pa.gdims[0]) : 0;
+      const int my = live ? nonneg_mod(pa.mom[p][1], pa.gdims[1]) : 0;
+      const int mz = live ? nonneg_mod(pa.mom[p][2], pa.gdims[2]) : 0;
+      const int kx = (mx * ((xs * xr + pa.origin[0]) % pa.gdims[0])) % pa.gdims[0];
+      const int ky = (my * ((y + pa.origin[1]) % pa.gdims[1])) % pa.gdims[1];
+      const int kz = (mz * ((z + pa.origin[2]) % pa.gdims[2])) % pa.gdims[2];
       k[tt] = kx | ky << 9 | kz << 18;
       m[tt] = mx | my << 9 | mz << 18;
     }
@@ -338,38 +340,44 @@ template <int NT> struct PhaseWalk {
     int kx = k[tt] & 511;
     if (pa.order && ((pbase + y + z) & 1)) {
       kx += m[tt] & 511;
-      kx -= kx >= pa.dims[0] ? pa.dims[0] : 0;
+      kx -= kx >= pa.gdims[0] ? pa.gdims[0] : 0;
     }
-    return cmul(cmul(tab[kx], tab[pa.dims[0] + ((k[tt] >> 9) & 511)]),
-                tab[pa.dims[0] + pa.dims[1] + (k[tt] >> 18)]);
+    return cmul(cmul(tab[kx], tab[pa.gdims[0] + ((k[tt] >> 9) & 511)]),
+                tab[pa.gdims[0] + pa.gdims[1] + (k[tt] >> 18)]);
   }
 
   __device__ __forceinline__ void advance(const ProjectArgs &pa)
   {
-    const int xstep = (pa.order ? 2 : 1) * SITES_PER_WAVE;
+    const int xs = pa.order ? 2 : 1;
+    const int Gx = pa.gdims[0], Gy = pa.gdims[1], Gz = pa.gdims[2];
 #pragma unroll
     for (int tt = 0; tt < NT; ++tt) {
-      const int kx = mod_small((k[tt] & 511) + xstep * (m[tt] & 511), pa.dims[0], inv_lx);
+      const int kx = mod_small((k[tt] & 511) + xs * SITES_PER_WAVE * (m[tt] & 511), Gx, inv_gx);
       k[tt] = (k[tt] & ~511) | kx;
     }
     xr += SITES_PER_WAVE;
     while (xr >= Lr) {
+      // x steps back by Lx: kx -= p_x Lx (0 mod Gx for a whole lattice), y steps on
       xr -= Lr;
       ++y;
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
+        const int mx = m[tt] & 511;
+        const int kx = mod_small((k[tt] & 511) + Gx - mod_small(mx * pa.dims[0], Gx, inv_gx), Gx, inv_gx);
         int ky = ((k[tt] >> 9) & 511) + ((m[tt] >> 9) & 511);
-        ky -= ky >= pa.dims[1] ? pa.dims[1] : 0;
-        k[tt] = (k[tt] & ~(511 << 9)) | ky << 9;
+        ky -= ky >= Gy ? Gy : 0;
+        k[tt] = (k[tt] & ~((1 << 18) - 1)) | ky << 9 | kx;
       }
       if (y == pa.dims[1]) {
         y = 0;
         ++z;
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) {
+          const int my = (m[tt] >> 9) & 511;
+          const int ky = mod_small(((k[tt] >> 9) & 511) + Gy - mod_small(my * pa.dims[1], Gy, inv_gy), Gy, inv_gy);
           int kz = (k[tt] >> 18) + (m[tt] >> 18);
-          kz -= kz >= pa.dims[2] ? pa.dims[2] : 0;
-          k[tt] = (k[tt] & ((1 << 18) - 1)) | kz << 18;
+          kz -= kz >= Gz ? Gz : 0;
+          k[tt] = (k[tt] & 511) | ky << 9 | kz << 18;
         }
       }
     }
@@ -402,14 +410,16 @@ __device__ __forceinline__ void project_accumulate(dbl4 (&acc_re)[NT], dbl4 (&ac
   }
 }
 
-// Build the phase tables exp(-2 pi i k / L_mu) of the three spatial directions
-// in LDS ([Lx | Ly | Lz] entries); a few sincospi per lane per work item.
+// Build the phase tables exp(-2 pi i k / G_mu) of the three spatial directions
+// (global extents) in LDS ([Gx | Gy | Gz] entries); a few sincospi per lane per
+// work item.
 __device__ __forceinline__ void load_phase_tab(double2 *tab, const ProjectArgs &pa, int lane)
 {
-  const int n = pa.dims[0] + pa.dims[1] + pa.dims[2];
+  const int G0 = pa.gdims[0], G1 = pa.gdims[1], G2 = pa.gdims[2];
+  const int n = G0 + G1 + G2;
   for (int e = lane; e < n; e += WAVE) {
-    const int L = e < pa.dims[0] ? pa.dims[0] : (e < pa.dims[0] + pa.dims[1] ? pa.dims[1] : pa.dims[2]);
-    const int k = e < pa.dims[0] ? e : (e < pa.dims[0] + pa.dims[1] ? e - pa.dims[0] : e - pa.dims[0] - pa.dims[1]);
+    const int L = e < G0 ? G0 : (e < G0 + G1 ? G1 : G2);
+    const int k = e < G0 ? e : (e < G0 + G1 ? e - G0 : e - G0 - G1);
     double sn, cs;
     sincospi(2.0 * (double)k / (double)L, &sn, &cs);
     tab[e] = make_double2(cs, -sn);
@@ -546,7 +556,7 @@ static void launch_project_g(const BaryonProjectArgs &a, const ProjectArgs &pa, 
 {
   const int nt = (pa.np + 15) / 16;
   const dim3 grid((unsigned)items), block(WAVE);
-  const size_t tab_bytes = sizeof(double2) * (pa.dims[0] + pa.dims[1] + pa.dims[2]);
+  const size_t tab_bytes = sizeof(double2) * (pa.gdims[0] + pa.gdims[1] + pa.gdims[2]);
   switch (nt) {
   case 1: hipLaunchKernelGGL((baryon_project_kernel<1, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
   case 2: hipLaunchKernelGGL((baryon_project_kernel<2, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;

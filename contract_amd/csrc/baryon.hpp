@@ -36,6 +36,8 @@ void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream);
 // ---- timeslice / momentum projection shared by the fused kernels --------
 struct ProjectArgs {
   int dims[4]; // Lx, Ly, Lz, Lt
+  int gdims[3];  // global spatial extents (phases are exp(-2 pi i p (x + origin) / gdims))
+  int origin[3]; // global coordinates of local site 0
   int order;   // 0 lexicographic [t][z][y][x]; 1 even-odd [parity][t][z][y][x/2]
   int np;
   int mom[MAX_MOM][3];

@@ -113,11 +113,44 @@ int check_momenta(const int *momenta, int nmom)
   return CONTRACT_OK;
 }
 
-void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const int dims[4], int order, const int *momenta,
-                       int nmom, void *ws)
+int resolve_geometry(Geometry &g, const int dims[4], int order, const int *global, const int *origin)
+{
+  int rc = check_lattice(dims, order);
+  if (rc != CONTRACT_OK) return rc;
+  for (int mu = 0; mu < 4; ++mu) g.dims[mu] = dims[mu];
+  g.order = order;
+  int tab = 0;
+  for (int mu = 0; mu < 3; ++mu) {
+    g.global[mu] = (global && global[mu] > 0) ? global[mu] : dims[mu];
+    g.origin[mu] = origin ? origin[mu] : 0;
+    if (g.global[mu] < dims[mu] || g.origin[mu] < 0 || g.origin[mu] >= g.global[mu]) {
+      set_last_error("direction %d: global extent %d and origin %d do not hold a local extent of %d", mu,
+                     g.global[mu], g.origin[mu], dims[mu]);
+      return CONTRACT_ERR_ARG;
+    }
+    if (order == 1 && (g.origin[mu] & 1)) {
+      set_last_error("even-odd order needs even origins (direction %d has %d)", mu, g.origin[mu]);
+      return CONTRACT_ERR_ARG;
+    }
+    tab += g.global[mu];
+  }
+  if (tab > MAX_PHASE_TAB) {
+    set_last_error("global Lx + Ly + Lz = %d exceeds %d", tab, MAX_PHASE_TAB);
+    return CONTRACT_ERR_ARG;
+  }
+  return CONTRACT_OK;
+}
+
+void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry &geo, const int *momenta, int nmom,
+                       void *ws)
 {
   std::memset(&pa, 0, sizeof(pa));
-  for (int mu = 0; mu < 4; ++mu) pa.dims[mu] = dims[mu];
+  for (int mu = 0; mu < 4; ++mu) pa.dims[mu] = geo.dims[mu];
+  for (int mu = 0; mu < 3; ++mu) {
+    pa.gdims[mu] = geo.global[mu];
+    pa.origin[mu] = geo.origin[mu];
+  }
+  const int order = geo.order;
   pa.order = order;
   pa.np = nmom;
   for (int p = 0; p < nmom; ++p)
@@ -208,16 +241,41 @@ int contract_baryon_twopoint_project(void *out, const void *propag_i, const void
                                      int precision, const int *contract_types, const double _Complex *weights,
                                      int nterm, int gamma_ij, int gamma_kl, const double _Complex *projectors,
                                      int nproj, const int *momenta, int nmom, const int dims[4], int site_order,
-                                     void *workspace, size_t workspace_bytes, void *stream_)
+                                     void *workspace, size_t workspace_bytes, void *stream)
+{
+  if (!dims) {
+    set_last_error("dims is NULL");
+    return CONTRACT_ERR_ARG;
+  }
+  contract_sublattice lat;
+  std::memset(&lat, 0, sizeof(lat));
+  for (int mu = 0; mu < 4; ++mu) lat.dims[mu] = dims[mu];
+  lat.site_order = site_order;
+  return contract_baryon_twopoint_project_sub(out, propag_i, propag_j, propag_n, precision, contract_types, weights,
+                                              nterm, gamma_ij, gamma_kl, projectors, nproj, momenta, nmom, &lat,
+                                              workspace, workspace_bytes, stream);
+}
+
+int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const void *propag_j,
+                                         const void *propag_n, int precision, const int *contract_types,
+                                         const double _Complex *weights, int nterm, int gamma_ij, int gamma_kl,
+                                         const double _Complex *projectors, int nproj, const int *momenta, int nmom,
+                                         const contract_sublattice *lattice, void *workspace, size_t workspace_bytes,
+                                         void *stream_)
 {
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   int rc;
-  if ((rc = check_lattice(dims, site_order)) != CONTRACT_OK) return rc;
-  if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
-  if (dims[0] + dims[1] + dims[2] > MAX_PHASE_TAB) {
-    set_last_error("Lx + Ly + Lz = %d exceeds %d", dims[0] + dims[1] + dims[2], MAX_PHASE_TAB);
+  if (!lattice) {
+    set_last_error("lattice is NULL");
     return CONTRACT_ERR_ARG;
   }
+  Geometry geo;
+  if ((rc = resolve_geometry(geo, lattice->dims, lattice->site_order, lattice->global, lattice->origin)) !=
+      CONTRACT_OK)
+    return rc;
+  const int *dims = geo.dims;
+  const int site_order = geo.order;
+  if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
   if (!out || !propag_i || !propag_j || !propag_n) {
     set_last_error("NULL output or propagator pointer");
     return CONTRACT_ERR_ARG;
@@ -258,7 +316,7 @@ int contract_baryon_twopoint_project(void *out, const void *propag_i, const void
   }
 
   ProjectArgs pa;
-  fill_project_args(pa, plan, dims, site_order, momenta, nmom, ws);
+  fill_project_args(pa, plan, geo, momenta, nmom, ws);
 
   BaryonProjectArgs a;
   std::memset(&a, 0, sizeof(a));

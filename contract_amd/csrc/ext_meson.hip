@@ -81,7 +81,8 @@ int check_requests(const contract_meson_request *reqs, int nreq)
     return CONTRACT_ERR_ARG;
   }
   for (int r = 0; r < nreq; ++r) {
-    int rc = check_lattice(reqs[r].dims, reqs[r].site_order);
+    Geometry geo;
+    int rc = resolve_geometry(geo, reqs[r].dims, reqs[r].site_order, reqs[r].global, reqs[r].origin);
     if (rc != CONTRACT_OK) return rc;
     const int *d = reqs[r].dims;
     if (d[0] + d[1] + d[2] > MAX_PHASE_TAB) {
@@ -186,7 +187,13 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
       MesonGridReq &R = a.req[r];
       R.propag_i = q.propag_i;
       R.propag_j = q.propag_j;
+      Geometry geo;
+      resolve_geometry(geo, q.dims, q.site_order, q.global, q.origin); // validated above
       for (int mu = 0; mu < 4; ++mu) R.dims[mu] = q.dims[mu];
+      for (int mu = 0; mu < 3; ++mu) {
+        R.gdims[mu] = geo.global[mu];
+        R.origin[mu] = geo.origin[mu];
+      }
       R.order = q.site_order;
       R.chunks_per_slice = p.chunks_per_slice;
       R.slice_sites = p.slice_sites;

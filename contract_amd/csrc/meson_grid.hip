@@ -100,21 +100,22 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
     last = last < s0 + R.slice_sites ? last : s0 + R.slice_sites;
 
     __syncthreads(); // the previous item's readers of tab are done
-    // per-momentum phase tables exp(-2 pi i p_mu k / L_mu), k < L_mu:
-    // [np][Lx + 1] | [np][Ly + 1] | [np][Lz + 1], so a lane's phase needs no
-    // division; the padding entry keeps rows of power-of-two length from
-    // starting on the same LDS bank (the eight rows a wave reads at once)
+    // per-momentum phase tables exp(-2 pi i p_mu (k + o_mu) / G_mu) for local
+    // k < L_mu (G, o: global extent and origin): [np][Lx + 1] | [np][Ly + 1] |
+    // [np][Lz + 1], so a lane's phase needs no division; the padding entry
+    // keeps rows of power-of-two length from starting on the same LDS bank
+    // (the eight rows a wave reads at once)
     const int Lx = R.dims[0], Ly = R.dims[1], Lz = R.dims[2];
     const int offy = a.np * (Lx + 1), offz = a.np * (Lx + Ly + 2);
     const int ntab = a.np * (Lx + Ly + Lz + 3);
     for (int e = tid; e < ntab; e += GRID_THREADS) {
       const int mu = e < offy ? 0 : (e < offz ? 1 : 2);
-      const int L = R.dims[mu];
+      const int L = R.dims[mu], G = R.gdims[mu];
       const int rel = e - (mu == 0 ? 0 : (mu == 1 ? offy : offz));
       const int p = rel / (L + 1), k = rel - p * (L + 1);
-      const int m = nonneg_mod(a.mom[p][mu], L);
+      const int m = nonneg_mod(a.mom[p][mu], G);
       double sn, cs;
-      sincospi(2.0 * (double)((m * k) % L) / (double)L, &sn, &cs);
+      sincospi(2.0 * (double)((m * ((k + R.origin[mu]) % G)) % G) / (double)G, &sn, &cs);
       tab[e] = make_double2(cs, -sn);
     }
     // this lane's MFMA row: momentum q >> 1, phase part q & 1 (0 re, 1 im)

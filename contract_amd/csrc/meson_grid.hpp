@@ -19,6 +19,7 @@ constexpr int GRID_PARTIAL = 16 * 2 * GRID_COLS; // doubles per work item: [row]
 struct MesonGridReq {
   const void *propag_i, *propag_j;
   int dims[4];
+  int gdims[3], origin[3]; // phases exp(-2 pi i p (x + origin) / gdims)
   int order;            // 0 lexicographic, 1 even-odd
   int chunks_per_slice; // work items per (parity, t) slice
   unsigned slice_sites;

@@ -28,8 +28,13 @@ struct ProjectPlan {
 ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item);
 int check_lattice(const int dims[4], int order);
 int check_momenta(const int *momenta, int nmom);
-void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const int dims[4], int order, const int *momenta,
-                       int nmom, void *ws);
+// Resolved sub-lattice geometry (global = dims, origin = 0 for a whole lattice).
+struct Geometry {
+  int dims[4], global[3], origin[3], order;
+};
+int resolve_geometry(Geometry &g, const int dims[4], int order, const int *global, const int *origin);
+void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry &geo, const int *momenta, int nmom,
+                       void *ws);
 void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const zhost *W, int nproj, void *out,
                        void *ws, hipStream_t stream);
 

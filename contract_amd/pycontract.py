@@ -236,15 +236,23 @@ def project_workspace_bytes(dims: Sequence[int], nmom: int, site_order: int = LE
                                                       int(nmom)))
 
 
+class Sublattice(ctypes.Structure):
+    """contract_sublattice."""
+    _fields_ = [("dims", ctypes.c_int * 4), ("global_", ctypes.c_int * 4), ("origin", ctypes.c_int * 4),
+                ("site_order", ctypes.c_int)]
+
+
 def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, gamma_kl: int, insertions,
                           momenta, dims, site_order: int = LEXICOGRAPHIC, out=None, workspace=None,
-                          stream=None):
+                          stream=None, global_dims=None, origin=None):
     """Fused baryon two-point: sum over ``terms`` = [(contract_type, weight)],
     for every insertion (gamma index or 4x4 projector), projected to
     ``out[len(insertions), len(momenta), Lt]`` (complex128, on the device).
 
     Propagators may be complex128 (FP64 path) or complex64 (FP32 contraction,
-    FP64 accumulation)."""
+    FP64 accumulation).  global_dims / origin place the local lattice inside a
+    larger one (one rank of an MPI grid): phases use global coordinates and
+    momenta are in units of 2pi / global extent."""
     import torch
     precision = CONTRACT_FP32 if propag_i.dtype == torch.complex64 else CONTRACT_FP64
     for p in (propag_i, propag_j, propag_n):
@@ -262,11 +270,24 @@ def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, ga
     if workspace is not None:
         ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
     vp = ctypes.c_void_p
-    rc = lib().contract_baryon_twopoint_project(
-        _ptr(out), _ptr(propag_i), _ptr(propag_j), _ptr(propag_n), precision, types.ctypes.data_as(vp),
-        weights.ctypes.data_as(vp), len(types), int(gamma_ij), int(gamma_kl), W.ctypes.data_as(vp), W.shape[0],
-        mom.ctypes.data_as(vp), mom.shape[0], d.ctypes.data_as(vp), int(site_order), ws_ptr, ws_bytes,
-        0 if stream is None else int(stream))
+    if global_dims is None and origin is None:
+        rc = lib().contract_baryon_twopoint_project(
+            _ptr(out), _ptr(propag_i), _ptr(propag_j), _ptr(propag_n), precision, types.ctypes.data_as(vp),
+            weights.ctypes.data_as(vp), len(types), int(gamma_ij), int(gamma_kl), W.ctypes.data_as(vp), W.shape[0],
+            mom.ctypes.data_as(vp), mom.shape[0], d.ctypes.data_as(vp), int(site_order), ws_ptr, ws_bytes,
+            0 if stream is None else int(stream))
+    else:
+        lat = Sublattice()
+        for mu in range(4):
+            lat.dims[mu] = int(d[mu])
+            lat.global_[mu] = int(global_dims[mu]) if global_dims is not None and mu < len(global_dims) else 0
+            lat.origin[mu] = int(origin[mu]) if origin is not None and mu < len(origin) else 0
+        lat.site_order = int(site_order)
+        rc = lib().contract_baryon_twopoint_project_sub(
+            _ptr(out), _ptr(propag_i), _ptr(propag_j), _ptr(propag_n), precision, types.ctypes.data_as(vp),
+            weights.ctypes.data_as(vp), len(types), int(gamma_ij), int(gamma_kl), W.ctypes.data_as(vp), W.shape[0],
+            mom.ctypes.data_as(vp), mom.shape[0], ctypes.byref(lat), ws_ptr, ws_bytes,
+            0 if stream is None else int(stream))
     if rc != 0:
         raise RuntimeError(f"contract_baryon_twopoint_project failed ({rc}): {last_error()}")
     return out
@@ -279,7 +300,20 @@ def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, ga
 class MesonRequest(ctypes.Structure):
     """contract_meson_request."""
     _fields_ = [("propag_i", ctypes.c_void_p), ("propag_j", ctypes.c_void_p), ("out", ctypes.c_void_p),
-                ("dims", ctypes.c_int * 4), ("site_order", ctypes.c_int)]
+                ("dims", ctypes.c_int * 4), ("site_order", ctypes.c_int), ("global_", ctypes.c_int * 3),
+                ("origin", ctypes.c_int * 3)]
+
+
+def _fill_geometry(r, req):
+    """Request tuple (Pi, Pj, dims[, site_order[, global_dims[, origin]]])."""
+    for mu in range(4):
+        r.dims[mu] = int(req[2][mu])
+    r.site_order = int(req[3]) if len(req) > 3 else LEXICOGRAPHIC
+    gl = req[4] if len(req) > 4 and req[4] is not None else (0, 0, 0)
+    og = req[5] if len(req) > 5 and req[5] is not None else (0, 0, 0)
+    for mu in range(3):
+        r.global_[mu] = int(gl[mu])
+        r.origin[mu] = int(og[mu])
 
 
 class MesonGridTuning(ctypes.Structure):
@@ -301,14 +335,10 @@ def _tuning(tuning):
 def _request_array(requests, outs):
     arr = (MesonRequest * len(requests))()
     for r, (req, out) in enumerate(zip(requests, outs)):
-        pi, pj, dims = req[0], req[1], req[2]
-        order = int(req[3]) if len(req) > 3 else LEXICOGRAPHIC
-        arr[r].propag_i = _ptr(pi)
-        arr[r].propag_j = _ptr(pj)
+        arr[r].propag_i = _ptr(req[0])
+        arr[r].propag_j = _ptr(req[1])
         arr[r].out = _ptr(out)
-        for mu in range(4):
-            arr[r].dims[mu] = int(dims[mu])
-        arr[r].site_order = order
+        _fill_geometry(arr[r], req)
     return arr
 
 
@@ -318,9 +348,7 @@ def meson_grid_workspace_bytes(requests, nmom: int, tuning=None) -> int:
     arr = (MesonRequest * len(requests))()
     for r, req in enumerate(requests):
         arr[r].propag_i = arr[r].propag_j = arr[r].out = 1
-        for mu in range(4):
-            arr[r].dims[mu] = int(req[2][mu])
-        arr[r].site_order = int(req[3]) if len(req) > 3 else LEXICOGRAPHIC
+        _fill_geometry(arr[r], req)
     t = _tuning(tuning)
     return int(lib().contract_meson_grid_workspace_bytes(arr, len(requests), int(nmom),
                                                          ctypes.byref(t) if t is not None else None))
@@ -328,7 +356,7 @@ def meson_grid_workspace_bytes(requests, nmom: int, tuning=None) -> int:
 
 def mesonGridProject(requests, momenta, tuning=None, outs=None, workspace=None, stream=None):
     """Batched meson two-point grids: for each request (propag_i, propag_j,
-    dims[, site_order]) return out[16 Gamma_ij][16 Gamma_kl][len(momenta)][Lt]
+    dims[, site_order[, global_dims[, origin]]]) return out[16 Gamma_ij][16 Gamma_kl][len(momenta)][Lt]
     (complex128 on the device) with out[g1][g2][p][t] = sum_{x in t}
     exp(-i p.x) meson_two_point(propag_i, propag_j; g1, g2)(x).
 

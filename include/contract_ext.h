@@ -67,6 +67,26 @@ int contract_baryon_twopoint_project(void *out, const void *propag_i, const void
                                      int nproj, const int *momenta, int nmom, const int dims[4], int site_order,
                                      void *workspace, size_t workspace_bytes, void *stream);
 
+/* The local lattice as one rank's part of a larger one (PyQUDA's MPI grid):
+ * momenta are in units of 2pi / global[mu] and the phase of local site x is
+ * exp(-i 2pi sum_mu p_mu (x_mu + origin[mu]) / global[mu]); out keeps the local
+ * timeslices (global timeslice = t + origin[3]).  A zero global[mu] means
+ * global[mu] = dims[mu].  Even-odd order needs even origins (local parity =
+ * global parity).  The plain entry point is the case global = dims, origin = 0. */
+typedef struct {
+  int dims[4];   /* local Lx, Ly, Lz, Lt */
+  int global[4]; /* global extents, >= dims; 0 = dims */
+  int origin[4]; /* global coordinates of local site (0, 0, 0, 0), 0 <= origin < global */
+  int site_order;
+} contract_sublattice;
+
+int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const void *propag_j,
+                                         const void *propag_n, int precision, const int *contract_types,
+                                         const double _Complex *weights, int nterm, int gamma_ij, int gamma_kl,
+                                         const double _Complex *projectors, int nproj, const int *momenta, int nmom,
+                                         const contract_sublattice *lattice, void *workspace, size_t workspace_bytes,
+                                         void *stream);
+
 /* ---- batched meson Gamma x Gamma' grids -------------------------------
  * One request = two propagators on one lattice.  For every request the full
  * 16 x 16 grid of meson_two_point(propag_i, propag_j, volume, g1, g2) (the
@@ -85,6 +105,8 @@ typedef struct {
   void *out;            /* device complex128 [16][16][nmom][Lt] */
   int dims[4];          /* Lx, Ly, Lz, Lt */
   int site_order;       /* CONTRACT_LEXICOGRAPHIC or CONTRACT_EVEN_ODD */
+  int global[3];        /* spatial global extents, 0 = dims (see contract_sublattice) */
+  int origin[3];        /* spatial global coordinates of local site 0 */
 } contract_meson_request;
 
 /* Work decomposition; a zero field takes the library default. */

@@ -54,8 +54,8 @@ def lib():
         L.oracle_baryon_general_two_point.argtypes = [P, P, P, P, I, S, I, I, P]
         L.oracle_baryon_sequential_two_point.argtypes = [P, P, P, I, I, S, I, I, I]
         L.oracle_site_coords.argtypes = [S, P, I, P]
-        L.oracle_project.argtypes = [P, P, I, P, I, P, I]
-        L.oracle_baryon_twopoint_project.argtypes = [P, P, P, P, P, P, I, I, I, P, I, P, I, P, I]
+        L.oracle_project.argtypes = [P, P, I, P, I, P, I, P, P]
+        L.oracle_baryon_twopoint_project.argtypes = [P, P, P, P, P, P, I, I, I, P, I, P, I, P, I, P, P]
         L.oracle_num_threads.restype = I
         _lib = L
     return _lib
@@ -180,8 +180,14 @@ def site_coords(V_dims, order: int = 0) -> np.ndarray:
     return out
 
 
-def project(site_corr, dims, momenta, order: int = 0) -> np.ndarray:
-    """Timeslice + momentum projection: site_corr [V, nq] -> [nq, np, Lt]."""
+def _geom(v):
+    return None if v is None else np.ascontiguousarray(list(v)[:3], dtype=np.int32)
+
+
+def project(site_corr, dims, momenta, order: int = 0, global_dims=None, origin=None) -> np.ndarray:
+    """Timeslice + momentum projection: site_corr [V, nq] -> [nq, np, Lt].
+    global_dims / origin: the local lattice as part of a larger one (phases in
+    global coordinates, momenta in units of 2pi / global extent)."""
     c = _c128(site_corr)
     if c.ndim == 1:
         c = c[:, None].copy()
@@ -189,12 +195,14 @@ def project(site_corr, dims, momenta, order: int = 0) -> np.ndarray:
     d = np.ascontiguousarray(dims, dtype=np.int32)
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
     out = np.zeros((nq, mom.shape[0], int(d[3])), np.complex128)
-    lib().oracle_project(_ptr(out), _ptr(c), nq, _ptr(d), order, _ptr(mom), mom.shape[0])
+    gl, og = _geom(global_dims), _geom(origin)
+    lib().oracle_project(_ptr(out), _ptr(c), nq, _ptr(d), order, _ptr(mom), mom.shape[0],
+                         None if gl is None else _ptr(gl), None if og is None else _ptr(og))
     return out
 
 
 def baryon_twopoint_project(propag_i, propag_j, propag_n, terms, gamma_ij: int, gamma_kl: int, insertions, dims,
-                            momenta, order: int = 0) -> np.ndarray:
+                            momenta, order: int = 0, global_dims=None, origin=None) -> np.ndarray:
     """Fused workload: sum_terms w * baryon_general_two_point(type, W) for each
     insertion W (a gamma index g -> G(g), or a 4x4 projector), projected to
     [len(insertions), len(momenta), Lt]."""
@@ -206,13 +214,15 @@ def baryon_twopoint_project(propag_i, propag_j, propag_n, terms, gamma_ij: int, 
     d = np.ascontiguousarray(dims, dtype=np.int32)
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
     out = np.zeros((W.shape[0], mom.shape[0], int(d[3])), np.complex128)
+    gl, og = _geom(global_dims), _geom(origin)
     lib().oracle_baryon_twopoint_project(_ptr(out), _ptr(pi), _ptr(pj), _ptr(pn), _ptr(types), _ptr(w), len(types),
                                          gamma_ij, gamma_kl, _ptr(W), W.shape[0], _ptr(d), order, _ptr(mom),
-                                         mom.shape[0])
+                                         mom.shape[0], None if gl is None else _ptr(gl),
+                                         None if og is None else _ptr(og))
     return out
 
 
-def meson_grid_project(propag_i, propag_j, dims, momenta, order: int = 0) -> np.ndarray:
+def meson_grid_project(propag_i, propag_j, dims, momenta, order: int = 0, global_dims=None, origin=None) -> np.ndarray:
     """The batched meson grid of one request: out[g1][g2][p][t] = projection
     of meson_two_point(propag_i, propag_j; gamma_ij = g1, gamma_kl = g2)
     (include/contract.h:16, include/meson.cuh:32-50), one oracle call per
@@ -223,7 +233,7 @@ def meson_grid_project(propag_i, propag_j, dims, momenta, order: int = 0) -> np.
     for g1 in range(16):
         for g2 in range(16):
             corr[:, g1 * 16 + g2] = meson_two_point(pi, pj, g1, g2)
-    out = project(corr, dims, momenta, order)
+    out = project(corr, dims, momenta, order, global_dims, origin)
     return out.reshape(16, 16, out.shape[1], out.shape[2])
 
 

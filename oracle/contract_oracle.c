@@ -437,22 +437,30 @@ static void neumaier_add(double *sum, double *comp, double v)
   *sum = t;
 }
 
-/* out[q][p][t] = sum_{x in timeslice t} exp(-i 2pi sum_mu p_mu x_mu / L_mu) c_q(x)
+/* out[q][p][t] = sum_{x in timeslice t} exp(-i 2pi sum_mu p_mu X_mu / G_mu) c_q(x)
  * for q < nq site correlators c given as site_corr[V][nq], momenta mom[np][3]
- * (integer components, any sign).  The phase of each direction is taken from
- * a table w_mu[k] = exp(-i 2pi k / L_mu) with k = p_mu x_mu mod L_mu, and the
- * three factors are multiplied x, then y, then z. */
-void oracle_project(double *out, const double *site_corr, int nq, const int dims[4], int order, const int *mom, int np)
+ * (integer components, any sign), X = x + origin the global coordinate and G
+ * the global extent (global = NULL or 0: G = L; origin = NULL: 0).  The phase
+ * of each direction is taken from a table w_mu[k] = exp(-i 2pi k / G_mu) with
+ * k = p_mu X_mu mod G_mu, and the three factors are multiplied x, then y, then
+ * z. */
+void oracle_project(double *out, const double *site_corr, int nq, const int dims[4], int order, const int *mom, int np,
+                    const int *global, const int *origin)
 {
   const int Lt = dims[3];
   const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  int G[3], O[3];
+  for (int mu = 0; mu < 3; ++mu) {
+    G[mu] = (global && global[mu] > 0) ? global[mu] : dims[mu];
+    O[mu] = origin ? origin[mu] : 0;
+  }
   const cplx *c = (const cplx *)site_corr;
   double *sum = (double *)calloc((size_t)nq * np * Lt * 4, sizeof(double));
   cplx *tab[3];
   for (int mu = 0; mu < 3; ++mu) {
-    tab[mu] = (cplx *)malloc(sizeof(cplx) * dims[mu]);
-    for (int k = 0; k < dims[mu]; ++k) {
-      const double th = 2.0 * M_PI * (double)k / (double)dims[mu];
+    tab[mu] = (cplx *)malloc(sizeof(cplx) * G[mu]);
+    for (int k = 0; k < G[mu]; ++k) {
+      const double th = 2.0 * M_PI * (double)k / (double)G[mu];
       tab[mu][k] = cx(cos(th), -sin(th));
     }
   }
@@ -462,8 +470,8 @@ void oracle_project(double *out, const double *site_corr, int nq, const int dims
     for (int p = 0; p < np; ++p) {
       cplx ph = cx(1, 0);
       for (int mu = 0; mu < 3; ++mu) {
-        long k = ((long)mom[p * 3 + mu] * co[mu]) % dims[mu];
-        if (k < 0) k += dims[mu];
+        long k = ((long)mom[p * 3 + mu] * (co[mu] + O[mu])) % G[mu];
+        if (k < 0) k += G[mu];
         ph = (mu == 0) ? tab[0][k] : cmul(ph, tab[mu][k]);
       }
       for (int q = 0; q < nq; ++q) {
@@ -489,7 +497,7 @@ void oracle_project(double *out, const double *site_corr, int nq, const int dims
 void oracle_baryon_twopoint_project(double *out, const double *propag_i, const double *propag_j,
                                     const double *propag_n, const int *types, const double *weights, int nterm,
                                     int gamma_ij, int gamma_kl, const double *projectors, int nproj, const int dims[4],
-                                    int order, const int *mom, int np)
+                                    int order, const int *mom, int np, const int *global, const int *origin)
 {
   const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
   double *site = (double *)calloc(V * nproj * 2, sizeof(double));
@@ -505,7 +513,7 @@ void oracle_baryon_twopoint_project(double *out, const double *propag_i, const d
         site[(x * nproj + w) * 2 + 1] += wr * b + wi * a;
       }
     }
-  oracle_project(out, site, nproj, dims, order, mom, np);
+  oracle_project(out, site, nproj, dims, order, mom, np, global, origin);
   free(tmp);
   free(site);
 }

@@ -183,3 +183,14 @@ def test_graph_replay(dev):
         torch.cuda.synchronize()
         for o, e in zip(outs, eager):
             assert np.array_equal(o.cpu().numpy(), e), rep
+
+
+def test_sublattice_geometry(dev):
+    torch, pc, _ = dev
+    dims, G, o = (4, 2, 4, 3), (8, 6, 4), (5, 3, 1)
+    a = oracle.synthetic_propagator(dims, 91, 0.3, t0=1)
+    b = oracle.synthetic_propagator(dims, 92, 0.3, t0=1)
+    req = (torch.from_numpy(a).to(dev[2]), torch.from_numpy(b).to(dev[2]), dims, 0, G, o)
+    got = pc.mesonGridProject([req], MOM7 + [(2, -1, 3)])[0].cpu().numpy()
+    want = oracle.meson_grid_project(a, b, dims, MOM7 + [(2, -1, 3)], 0, G, o)
+    assert grid_error(got, want) < 1e-12

@@ -107,3 +107,36 @@ def test_caller_workspace_and_repeat(dev):
     a = pc.baryonTwoPointProject(*d, PROTON, 5, 5, [0, 8], MOM7, dims, workspace=ws).cpu().numpy()
     b = pc.baryonTwoPointProject(*d, PROTON, 5, 5, [0, 8], MOM7, dims).cpu().numpy()
     assert np.array_equal(a, b)  # deterministic reduction order
+
+
+@pytest.mark.parametrize("order,origin", [(0, (3, 1, 2)), (1, (2, 4, 0))])
+def test_sublattice_geometry(dev, order, origin):
+    """Local 4x4x2x4 lattice at `origin` inside a 10x6x4 spatial global lattice:
+    phases in global coordinates, momenta in units of 2pi / global extent."""
+    torch, pc, cuda = dev
+    dims, G = (4, 4, 2, 4), (10, 6, 4)
+    ps = [oracle.synthetic_propagator(dims, s, 0.3, t0=1) for s in (71, 72, 73)]
+    d = [torch.from_numpy(p).to(cuda) for p in ps]
+    moms = MOM27[:9] + [(3, -2, 1)]
+    got = pc.baryonTwoPointProject(*d, PROTON, 5, 5, [0, 8, 15], moms, dims, site_order=order, global_dims=G,
+                                   origin=origin).cpu().numpy()
+    want = oracle.baryon_twopoint_project(*ps, PROTON, 5, 5, [0, 8, 15], dims, moms, order=order, global_dims=G,
+                                          origin=origin)
+    assert per_timeslice_error(got, want) < 1e-12
+
+
+def test_sublattices_add_up_to_the_lattice(dev):
+    """An 8x4x4x4 lattice split along x into 2 + 6 sites: the two ranks'
+    projections add up to the whole lattice's."""
+    torch, pc, cuda = dev
+    dims = (8, 4, 4, 4)
+    ps = [oracle.synthetic_propagator(dims, s, 0.3, t0=1) for s in (81, 82, 83)]
+    whole = pc.baryonTwoPointProject(*[torch.from_numpy(p).to(cuda) for p in ps], PROTON, 5, 5, list(range(16)),
+                                     MOM27, dims).cpu().numpy()
+    total = np.zeros_like(whole)
+    for lo, hi in [(0, 2), (2, 8)]:
+        part = [np.ascontiguousarray(p.reshape(4, 4, 4, 8, 144)[:, :, :, lo:hi]).reshape(-1, 4, 4, 3, 3) for p in ps]
+        total += pc.baryonTwoPointProject(*[torch.from_numpy(p).to(cuda) for p in part], PROTON, 5, 5,
+                                          list(range(16)), MOM27, (hi - lo, 4, 4, 4), global_dims=dims[:3],
+                                          origin=(lo, 0, 0)).cpu().numpy()
+    assert per_timeslice_error(total, whole) < 1e-12
