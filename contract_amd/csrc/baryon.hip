@@ -489,7 +489,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT
   using C = cplx<Real>;
   constexpr int SL = site_lds<C>();
   __shared__ C lds[3][SITES_PER_WAVE * SL];
-  extern __shared__ double2 tab[]; // Lx + Ly + Lz phase entries, sized at launch
+  extern __shared__ double2 tab[]; // Gx + Gy + Gz phase entries, sized at launch
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
   const size_t item = blockIdx.x;
