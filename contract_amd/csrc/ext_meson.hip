@@ -102,6 +102,37 @@ int check_requests(const contract_meson_request *reqs, int nreq)
   return CONTRACT_OK;
 }
 
+// A projected momentum row pair serves k and -k: the rows hold sum cos(k.x) K
+// and -sum sin(k.x) K, and P(+-k) = A +- iB.  slots: one per distinct pair,
+// with the output index of +k and of -k (-1 if not requested).  Exact integer
+// negation; duplicates keep their first index (later copies get their own
+// slot).
+struct MomSlot {
+  int rep, plus, minus;
+};
+
+std::vector<MomSlot> pair_momenta(const int *mom, int nmom)
+{
+  std::vector<MomSlot> slots;
+  std::vector<char> used(nmom, 0);
+  for (int p = 0; p < nmom; ++p) {
+    if (used[p]) continue;
+    used[p] = 1;
+    MomSlot s{p, p, -1};
+    const bool zero = mom[3 * p] == 0 && mom[3 * p + 1] == 0 && mom[3 * p + 2] == 0;
+    if (!zero)
+      for (int q = p + 1; q < nmom; ++q)
+        if (!used[q] && mom[3 * q] == -mom[3 * p] && mom[3 * q + 1] == -mom[3 * p + 1] &&
+            mom[3 * q + 2] == -mom[3 * p + 2]) {
+          used[q] = 1;
+          s.minus = q;
+          break;
+        }
+    slots.push_back(s);
+  }
+  return slots;
+}
+
 // Largest item count of any launch group (GRID_MAX_REQ consecutive requests).
 size_t max_group_items(const contract_meson_request *reqs, int nreq, int chunk)
 {
@@ -164,6 +195,7 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
     own = true;
   }
 
+  const std::vector<MomSlot> slots = pair_momenta(momenta, nmom);
   MesonGridFinish f;
   std::memset(&f, 0, sizeof(f));
   for (int gm = 0; gm < 16; ++gm) f.g5t[gm] = sparse_of(mat_transpose(gamma5_times(gm)));
@@ -211,14 +243,17 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
     }
     a.total_items = (unsigned)items;
     const size_t grid = std::min(items, (size_t)device_cu_count() * g.bpc);
-    for (int p0 = 0; p0 < nmom; p0 += GRID_MOM_PER_PASS) {
-      a.np = std::min(nmom - p0, GRID_MOM_PER_PASS);
-      for (int p = 0; p < a.np; ++p)
-        for (int mu = 0; mu < 3; ++mu) a.mom[p][mu] = momenta[3 * (p0 + p) + mu];
+    for (size_t p0 = 0; p0 < slots.size(); p0 += GRID_MOM_PER_PASS) {
+      a.np = (int)std::min(slots.size() - p0, (size_t)GRID_MOM_PER_PASS);
+      for (int p = 0; p < a.np; ++p) {
+        const MomSlot &m = slots[p0 + p];
+        for (int mu = 0; mu < 3; ++mu) a.mom[p][mu] = momenta[3 * m.rep + mu];
+        f.out_plus[p] = m.plus;
+        f.out_minus[p] = m.minus;
+      }
       launch_meson_grid(a, precision == CONTRACT_FP32, g.tile, grid, (size_t)tab * a.np, stream);
       CONTRACT_EXT_HIP(hipGetLastError());
       f.np = a.np;
-      f.p_base = p0;
       launch_meson_grid_finish(f, blocks, stream);
       CONTRACT_EXT_HIP(hipGetLastError());
     }

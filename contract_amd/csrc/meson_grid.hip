@@ -264,19 +264,19 @@ __device__ __forceinline__ void nsum(double &s, double &c, double v)
   s = t;
 }
 
-// One block per (request, timeslice, momentum), 512 threads = two chunk
+// One block per (request, timeslice, momentum slot), 512 threads = two chunk
 // groups x one thread per K column: each thread sums every other work-item
 // partial of the timeslice in item order (Neumaier compensated), the two
 // groups are combined in a fixed order (so the result is bit-reproducible),
-// the (phase part, K part) rows are combined into the complex
-// momentum-projected K, and thread (g1, g2) applies the gamma pair.  Output
-// out[g1][g2][p_base + p][t].
+// and the rows A = sum cos(k.x) K, B = -sum sin(k.x) K give the projected K of
+// momentum +k (A + iB) and of -k (A - iB); thread (g1, g2) applies the gamma
+// pair to each requested sign.  Output out[g1][g2][momentum][t].
 constexpr int FINISH_GROUPS = 2;
 
 __global__ __launch_bounds__(FINISH_GROUPS * GRID_COLS) void meson_grid_finish_kernel(MesonGridFinish f)
 {
   __shared__ double part[FINISH_GROUPS][4][2][GRID_COLS];
-  __shared__ double2 kp[GRID_COLS];
+  __shared__ double2 kp[2][GRID_COLS];
   const int col = threadIdx.x % GRID_COLS, cg = threadIdx.x / GRID_COLS;
   const int p = blockIdx.y;
   int r = 0;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(FINISH_GROUPS * GRID_COLS) void meson_grid_finish_k
     const size_t slice = (size_t)par * R.lt + t;
     for (int ch = cg; ch < R.chunks_per_slice; ch += FINISH_GROUPS) {
       const double *src = f.partial + (R.item0 + slice * R.chunks_per_slice + ch) * (size_t)GRID_PARTIAL;
-      // row 2p: phase re, row 2p + 1: phase im; each with K re / K im
+      // row 2p: cos weight, row 2p + 1: -sin weight; each with K re / K im
 #pragma unroll
       for (int u = 0; u < 4; ++u) nsum(s[u], cc[u], src[(4 * p + u) * GRID_COLS + col]);
     }
@@ -311,21 +311,28 @@ __global__ __launch_bounds__(FINISH_GROUPS * GRID_COLS) void meson_grid_finish_k
       }
       v[u] = S + Cc;
     }
-    kp[col] = make_double2(v[0] - v[3], v[1] + v[2]); // (ph_re + i ph_im)(K_re + i K_im)
+    // A = (v0, v1), B = (v2, v3): A + iB and A - iB
+    kp[0][col] = make_double2(v[0] - v[3], v[1] + v[2]);
+    kp[1][col] = make_double2(v[0] + v[3], v[1] - v[2]);
   }
   __syncthreads();
   if (cg == 0) {
     const int g1 = col >> 4, g2 = col & 15;
     const SparseSpin a1 = f.g5t[g1], a2 = f.g5t[g2];
-    double2 acc = make_double2(0.0, 0.0);
-    for (int ii = 0; ii < 4; ++ii)
-      for (int l = 0; l < 4; ++l) {
-        const int jj = a1.col[ii], kk = a2.col[l];
-        const double2 wgt = cmul(make_double2(a1.re[ii], a1.im[ii]), make_double2(a2.re[l], a2.im[l]));
-        acc = cfma(acc, wgt, kp[(kk * 4 + l) * 16 + ii * 4 + jj]);
-      }
     double2 *out = static_cast<double2 *>(R.out);
-    out[((size_t)(g1 * 16 + g2) * f.np_total + f.p_base + p) * R.lt + t] = acc;
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const int po = sgn ? f.out_minus[p] : f.out_plus[p];
+      if (po < 0) continue;
+      double2 acc = make_double2(0.0, 0.0);
+      for (int ii = 0; ii < 4; ++ii)
+        for (int l = 0; l < 4; ++l) {
+          const int jj = a1.col[ii], kk = a2.col[l];
+          const double2 wgt = cmul(make_double2(a1.re[ii], a1.im[ii]), make_double2(a2.re[l], a2.im[l]));
+          acc = cfma(acc, wgt, kp[sgn][(kk * 4 + l) * 16 + ii * 4 + jj]);
+        }
+      out[((size_t)(g1 * 16 + g2) * f.np_total + po) * R.lt + t] = acc;
+    }
   }
 }
 

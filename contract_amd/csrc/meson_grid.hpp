@@ -45,7 +45,10 @@ struct MesonGridOut {
 
 struct MesonGridFinish {
   MesonGridOut req[GRID_MAX_REQ];
-  int nreq, np, np_total, p_base;
+  int nreq, np, np_total;
+  // slot p of the pass holds momentum k: output index of +k and of -k (the
+  // rows give both, P(+-k) = A +- iB), -1 where that sign was not requested
+  int out_plus[GRID_MOM_PER_PASS], out_minus[GRID_MOM_PER_PASS];
   const double *partial;
   SparseSpin g5t[16]; // (g5 G(g))^T: row i -> (j, (g5 G(g))[j][i])
 };
