@@ -384,29 +384,27 @@ template <int NT> struct PhaseWalk {
   }
 };
 
+// Momentum slot k (one per +-k pair) keeps C = sum cos(k.x) R and
+// S = -sum sin(k.x) R in four accumulators (C re, C im, S re, S im); the finish
+// forms P(+-k) = C +- iS.
 template <int NT>
-__device__ __forceinline__ void project_accumulate(dbl4 (&acc_re)[NT], dbl4 (&acc_im)[NT], const ProjectArgs &pa,
-                                                   const double2 *tab, double2 R, const PhaseWalk<NT> &walk)
+__device__ __forceinline__ void project_accumulate(dbl4 (&acc)[NT][4], const ProjectArgs &pa, const double2 *tab,
+                                                   double2 R, const PhaseWalk<NT> &walk)
 {
   double pr[NT], pi[NT];
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
-    const double2 ph = walk.phase(pa, tab, tt);
+    const double2 ph = walk.phase(pa, tab, tt); // (cos, -sin)
     const bool live = tt * 16 + (int)(threadIdx.x & 15) < pa.np; // rows past np stay zero
     pr[tt] = live ? ph.x : 0.0;
     pi[tt] = live ? ph.y : 0.0;
   }
-  // the two products into each accumulator are issued NT MFMAs apart, so a
-  // dependent MFMA rarely waits for the previous one to drain
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
-    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.x, acc_re[tt], 0, 0, 0);
-    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.y, acc_im[tt], 0, 0, 0);
-  }
-#pragma unroll
-  for (int tt = 0; tt < NT; ++tt) {
-    acc_re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-pi[tt], R.y, acc_re[tt], 0, 0, 0);
-    acc_im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[tt], R.x, acc_im[tt], 0, 0, 0);
+    acc[tt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.x, acc[tt][0], 0, 0, 0);
+    acc[tt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.y, acc[tt][1], 0, 0, 0);
+    acc[tt][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[tt], R.x, acc[tt][2], 0, 0, 0);
+    acc[tt][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[tt], R.y, acc[tt][3], 0, 0, 0);
   }
 }
 
@@ -427,17 +425,19 @@ __device__ __forceinline__ void load_phase_tab(double2 *tab, const ProjectArgs &
 }
 
 template <int NT>
-__device__ __forceinline__ void project_store(const dbl4 (&acc_re)[NT], const dbl4 (&acc_im)[NT],
-                                              const ProjectArgs &pa, size_t item, int lane)
+__device__ __forceinline__ void project_store(const dbl4 (&acc)[NT][4], const ProjectArgs &pa, size_t item, int lane)
 {
   const int g = lane >> 4, q = lane & 15;
-  double2 *dst = pa.partial + item * (size_t)pa.np * 16;
+  double2 *dst = pa.partial + item * (size_t)pa.np * 32;
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int p = tt * 16 + g + 4 * r; // v_mfma_f64_16x16x4f64: lane (g, q) register r holds D[g + 4r][q]
-      if (p < pa.np) dst[p * 16 + q] = make_double2(acc_re[tt][r], acc_im[tt][r]);
+      if (p < pa.np) {
+        dst[p * 32 + q] = make_double2(acc[tt][0][r], acc[tt][1][r]);      // C
+        dst[p * 32 + 16 + q] = make_double2(acc[tt][2][r], acc[tt][3][r]); // S
+      }
     }
 }
 
@@ -483,7 +483,8 @@ __device__ __forceinline__ int normalise_site(float2 *site_lds, int q)
 
 template <int NT, typename Real, int GIJ = -1>
 __global__ __launch_bounds__(WAVE)
-__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT_WPE32))) void baryon_project_kernel(
+__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : (NT == 1 ? CONTRACT_WPE32 : 1)))) void
+baryon_project_kernel(
   BaryonProjectArgs a, ProjectArgs pa)
 {
   using C = cplx<Real>;
@@ -500,9 +501,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT
   PhaseWalk<NT> walk;
   walk.init(pa, first, g, q);
 
-  dbl4 acc_re[NT], acc_im[NT];
+  dbl4 acc[NT][4];
 #pragma unroll
-  for (int tt = 0; tt < NT; ++tt) acc_re[tt] = acc_im[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[tt][u] = dbl4{0.0, 0.0, 0.0, 0.0};
 
   const C *P[3] = {(const C *)a.propag_i, (const C *)a.propag_j, (const C *)a.propag_n};
   auto nsites_of = [&](size_t s) {
@@ -544,11 +547,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : CONTRACT
       R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)r.x, (double)r.y));
     }
     if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
-    project_accumulate<NT>(acc_re, acc_im, pa, tab, R, walk); // padded lanes carry R = 0
+    project_accumulate<NT>(acc, pa, tab, R, walk); // padded lanes carry R = 0
     walk.advance(pa);
     __syncthreads();
   }
-  project_store<NT>(acc_re, acc_im, pa, item, lane);
+  project_store<NT>(acc, pa, item, lane);
 }
 
 template <typename Real, int GIJ>

@@ -44,7 +44,7 @@ struct ProjectArgs {
   int chunk;   // sites per work item (multiple of 4)
   int chunks_per_slice;
   size_t slice_sites; // sites per (parity, t) slice
-  double2 *partial;   // [items][np][16]
+  double2 *partial;   // [items][np slots][2][16]: C and S of each momentum slot
 };
 
 struct BaryonProjectArgs {
@@ -70,14 +70,19 @@ struct ReduceArgs {
 };
 void launch_project_reduce(const ReduceArgs &r, hipStream_t stream);
 
-// out[w][p][t] = sum_q W_w[q] open[t][p][q] for up to MAX_PROJ_PER_LAUNCH projectors.
+// open[t][slot][2][16] holds, per momentum slot k, C = sum cos(k.x) R and
+// S = -sum sin(k.x) R; requested momentum p is slot[p] with sign sgn[p]:
+// out[w][p][t] = sum_q W_w[q] (C[q] + sgn i S[q]), for up to
+// MAX_PROJ_PER_LAUNCH projectors per launch.
 struct ApplyArgs {
   const double2 *open;
-  int np, lt, nq;
+  int nslot, lt;
+  int nmom;      // requested momenta (outputs)
   int nproj;     // projectors in this launch
   int proj_base; // index of the first one in the output
+  int slot[MAX_MOM], sgn[MAX_MOM];
   double w_re[MAX_PROJ_PER_LAUNCH][16], w_im[MAX_PROJ_PER_LAUNCH][16];
-  double2 *out;  // [nproj_total][np][lt]
+  double2 *out;  // [nproj_total][nmom][lt]
 };
 void launch_project_apply(const ApplyArgs &f, hipStream_t stream);
 

@@ -55,6 +55,28 @@ size_t persistent_grid(size_t ntiles, int waves_per_cu)
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+std::vector<MomSlot> pair_momenta(const int *mom, int nmom)
+{
+  std::vector<MomSlot> slots;
+  std::vector<char> used(nmom, 0);
+  for (int p = 0; p < nmom; ++p) {
+    if (used[p]) continue;
+    used[p] = 1;
+    MomSlot s{p, p, -1};
+    const bool zero = mom[3 * p] == 0 && mom[3 * p + 1] == 0 && mom[3 * p + 2] == 0;
+    if (!zero)
+      for (int q = p + 1; q < nmom; ++q)
+        if (!used[q] && mom[3 * q] == -mom[3 * p] && mom[3 * q + 1] == -mom[3 * p + 1] &&
+            mom[3 * q + 2] == -mom[3 * p + 2]) {
+          used[q] = 1;
+          s.minus = q;
+          break;
+        }
+    slots.push_back(s);
+  }
+  return slots;
+}
+
 ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item)
 {
   ProjectPlan p{};
@@ -76,8 +98,8 @@ ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_i
   p.chunk = (int)chunk;
   p.chunks_per_slice = (int)((p.slice_sites + chunk - 1) / chunk);
   p.items = (size_t)p.slices * p.chunks_per_slice;
-  p.partial_bytes = align256(p.items * (size_t)nmom * 16 * sizeof(double2));
-  p.open_bytes = align256((size_t)dims[3] * nmom * 16 * sizeof(double2));
+  p.partial_bytes = align256(p.items * (size_t)nmom * 32 * sizeof(double2));
+  p.open_bytes = align256((size_t)dims[3] * nmom * 32 * sizeof(double2));
   p.total = p.partial_bytes + p.open_bytes;
   return p;
 }
@@ -141,9 +163,10 @@ int resolve_geometry(Geometry &g, const int dims[4], int order, const int *globa
   return CONTRACT_OK;
 }
 
-void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry &geo, const int *momenta, int nmom,
-                       void *ws)
+void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry &geo, const int *momenta,
+                       const std::vector<MomSlot> &slots, void *ws)
 {
+  const int nmom = (int)slots.size();
   std::memset(&pa, 0, sizeof(pa));
   for (int mu = 0; mu < 4; ++mu) pa.dims[mu] = geo.dims[mu];
   for (int mu = 0; mu < 3; ++mu) {
@@ -154,7 +177,7 @@ void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry 
   pa.order = order;
   pa.np = nmom;
   for (int p = 0; p < nmom; ++p)
-    for (int mu = 0; mu < 3; ++mu) pa.mom[p][mu] = momenta[3 * p + mu];
+    for (int mu = 0; mu < 3; ++mu) pa.mom[p][mu] = momenta[3 * slots[p].rep + mu];
   pa.chunk = plan.chunk;
   pa.chunks_per_slice = plan.chunks_per_slice;
   pa.slice_sites = plan.slice_sites;
@@ -162,16 +185,17 @@ void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry 
 }
 
 // Per-timeslice reduction of the work-item partials, then the projectors
-// (MAX_PROJ_PER_LAUNCH per launch).
-void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const zhost *W, int nproj, void *out,
-                       void *ws, hipStream_t stream)
+// (MAX_PROJ_PER_LAUNCH per launch), expanding each momentum slot to its +k
+// and -k outputs.
+void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const std::vector<MomSlot> &slots, int nmom,
+                       const zhost *W, int nproj, void *out, void *ws, hipStream_t stream)
 {
   ReduceArgs r;
   r.partial = pa.partial;
   r.open = reinterpret_cast<double2 *>(static_cast<char *>(ws) + plan.partial_bytes);
   r.np = pa.np;
   r.lt = pa.dims[3];
-  r.nq = 16;
+  r.nq = 32;
   r.order = pa.order;
   r.chunks_per_slice = plan.chunks_per_slice;
   launch_project_reduce(r, stream);
@@ -179,9 +203,17 @@ void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const zho
     ApplyArgs f;
     std::memset(&f, 0, sizeof(f));
     f.open = r.open;
-    f.np = pa.np;
+    f.nslot = pa.np;
     f.lt = pa.dims[3];
-    f.nq = 16;
+    f.nmom = nmom;
+    for (int k = 0; k < (int)slots.size(); ++k) {
+      f.slot[slots[k].plus] = k;
+      f.sgn[slots[k].plus] = 1;
+      if (slots[k].minus >= 0) {
+        f.slot[slots[k].minus] = k;
+        f.sgn[slots[k].minus] = -1;
+      }
+    }
     f.nproj = nproj - base < MAX_PROJ_PER_LAUNCH ? nproj - base : MAX_PROJ_PER_LAUNCH;
     f.proj_base = base;
     for (int w = 0; w < f.nproj; ++w)
@@ -302,7 +334,8 @@ int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const 
     return CONTRACT_ERR_ARG;
   }
 
-  const ProjectPlan plan = plan_project(dims, site_order, nmom, 0);
+  const std::vector<MomSlot> slots = pair_momenta(momenta, nmom);
+  const ProjectPlan plan = plan_project(dims, site_order, (int)slots.size(), 0);
   void *ws = workspace;
   bool own = false;
   if (ws) {
@@ -316,7 +349,7 @@ int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const 
   }
 
   ProjectArgs pa;
-  fill_project_args(pa, plan, geo, momenta, nmom, ws);
+  fill_project_args(pa, plan, geo, momenta, slots, ws);
 
   BaryonProjectArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -345,7 +378,7 @@ int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const 
   std::vector<zhost> W((size_t)nproj * 16);
   const double *pm = reinterpret_cast<const double *>(projectors);
   for (size_t e = 0; e < W.size(); ++e) W[e] = zhost(pm[2 * e], pm[2 * e + 1]);
-  launch_finish_all(pa, plan, W.data(), nproj, out, ws, stream);
+  launch_finish_all(pa, plan, slots, nmom, W.data(), nproj, out, ws, stream);
   CONTRACT_EXT_HIP(hipGetLastError());
   if (own) CONTRACT_EXT_HIP(hipFreeAsync(ws, stream));
   return CONTRACT_OK;

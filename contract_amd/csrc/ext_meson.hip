@@ -102,37 +102,6 @@ int check_requests(const contract_meson_request *reqs, int nreq)
   return CONTRACT_OK;
 }
 
-// A projected momentum row pair serves k and -k: the rows hold sum cos(k.x) K
-// and -sum sin(k.x) K, and P(+-k) = A +- iB.  slots: one per distinct pair,
-// with the output index of +k and of -k (-1 if not requested).  Exact integer
-// negation; duplicates keep their first index (later copies get their own
-// slot).
-struct MomSlot {
-  int rep, plus, minus;
-};
-
-std::vector<MomSlot> pair_momenta(const int *mom, int nmom)
-{
-  std::vector<MomSlot> slots;
-  std::vector<char> used(nmom, 0);
-  for (int p = 0; p < nmom; ++p) {
-    if (used[p]) continue;
-    used[p] = 1;
-    MomSlot s{p, p, -1};
-    const bool zero = mom[3 * p] == 0 && mom[3 * p + 1] == 0 && mom[3 * p + 2] == 0;
-    if (!zero)
-      for (int q = p + 1; q < nmom; ++q)
-        if (!used[q] && mom[3 * q] == -mom[3 * p] && mom[3 * q + 1] == -mom[3 * p + 1] &&
-            mom[3 * q + 2] == -mom[3 * p + 2]) {
-          used[q] = 1;
-          s.minus = q;
-          break;
-        }
-    slots.push_back(s);
-  }
-  return slots;
-}
-
 // Largest item count of any launch group (GRID_MAX_REQ consecutive requests).
 size_t max_group_items(const contract_meson_request *reqs, int nreq, int chunk)
 {

@@ -3,6 +3,8 @@
 
 #include <stddef.h>
 
+#include <vector>
+
 #include "baryon.hpp"
 #include "spin.hpp"
 
@@ -19,12 +21,22 @@ struct ProjectPlan {
   int chunk;
   int chunks_per_slice;
   size_t items;
-  size_t partial_bytes; // [items][nmom][16] complex
-  size_t open_bytes;    // [Lt][nmom][16] complex, the per-timeslice sums
+  size_t partial_bytes; // [items][nslot][2][16] complex
+  size_t open_bytes;    // [Lt][nslot][2][16] complex, the per-timeslice sums
   size_t total;
 };
 
-// sites_per_item <= 0 picks ~16 items per compute unit.
+// A momentum k and its negative share one slot: the projection keeps
+// C = sum cos(k.x) c(x) and S = -sum sin(k.x) c(x), and P(+-k) = C +- iS.
+// One slot per distinct pair (exact integer negation; the zero momentum and
+// duplicates get their own), with the output index of +k and -k (-1 if not
+// requested).
+struct MomSlot {
+  int rep, plus, minus;
+};
+std::vector<MomSlot> pair_momenta(const int *mom, int nmom);
+
+// sites_per_item <= 0 picks ~16 items per compute unit; nslot momentum slots.
 ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item);
 int check_lattice(const int dims[4], int order);
 int check_momenta(const int *momenta, int nmom);
@@ -33,9 +45,9 @@ struct Geometry {
   int dims[4], global[3], origin[3], order;
 };
 int resolve_geometry(Geometry &g, const int dims[4], int order, const int *global, const int *origin);
-void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry &geo, const int *momenta, int nmom,
-                       void *ws);
-void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const zhost *W, int nproj, void *out,
-                       void *ws, hipStream_t stream);
+void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry &geo, const int *momenta,
+                       const std::vector<MomSlot> &slots, void *ws);
+void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const std::vector<MomSlot> &slots, int nmom,
+                       const zhost *W, int nproj, void *out, void *ws, hipStream_t stream);
 
 } // namespace contract_amd

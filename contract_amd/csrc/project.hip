@@ -14,16 +14,18 @@ __device__ __forceinline__ void neumaier(double &sum, double &comp, double v)
   sum = t;
 }
 
-// One block of 256 threads per (momentum p, timeslice t): thread (c0, q)
-// sums component q over the work items c0, c0 + 16, ... of the timeslice
-// (both parities for even-odd order); the 16 column partials are then
-// combined in index order.  Every sum is Neumaier compensated and the order
-// is fixed, so results are bit-reproducible.
-__global__ __launch_bounds__(256) void project_reduce_kernel(ReduceArgs r)
+// One block of 512 threads per (momentum slot p, timeslice t): thread (c0, q)
+// sums component q (< nq <= 32) over the work items c0, c0 + 16, ... of the
+// timeslice (both parities for even-odd order); the 16 column partials are
+// then combined in index order.  Every sum is Neumaier compensated and the
+// order is fixed, so results are bit-reproducible.
+constexpr int REDUCE_Q = 32;
+
+__global__ __launch_bounds__(16 * REDUCE_Q) void project_reduce_kernel(ReduceArgs r)
 {
-  __shared__ double part[16][16][4];
+  __shared__ double part[16][REDUCE_Q][4];
   const int p = blockIdx.x, t = blockIdx.y;
-  const int q = threadIdx.x & 15, c0 = threadIdx.x >> 4;
+  const int q = threadIdx.x % REDUCE_Q, c0 = threadIdx.x / REDUCE_Q;
   double sr = 0, cr = 0, si = 0, ci = 0;
   if (q < r.nq) {
     const int nslices = r.order ? 2 : 1;
@@ -56,26 +58,30 @@ __global__ __launch_bounds__(256) void project_reduce_kernel(ReduceArgs r)
 
 void launch_project_reduce(const ReduceArgs &r, hipStream_t stream)
 {
-  hipLaunchKernelGGL(project_reduce_kernel, dim3(r.np, r.lt), dim3(256), 0, stream, r);
+  hipLaunchKernelGGL(project_reduce_kernel, dim3(r.np, r.lt), dim3(16 * REDUCE_Q), 0, stream, r);
 }
 
 __global__ __launch_bounds__(256) void project_apply_kernel(ApplyArgs f)
 {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; // (w, p, t)
-  const size_t total = (size_t)f.nproj * f.np * f.lt;
+  const size_t total = (size_t)f.nproj * f.nmom * f.lt;
   if (e >= total) return;
   const int t = (int)(e % f.lt);
-  const int p = (int)((e / f.lt) % f.np);
-  const int w = (int)(e / ((size_t)f.lt * f.np));
-  const double2 *o = f.open + ((size_t)t * f.np + p) * f.nq;
+  const int p = (int)((e / f.lt) % f.nmom);
+  const int w = (int)(e / ((size_t)f.lt * f.nmom));
+  const double2 *o = f.open + ((size_t)t * f.nslot + f.slot[p]) * 32; // C[16], then S[16]
+  const double sg = (double)f.sgn[p];
   double2 acc = make_double2(0.0, 0.0);
-  for (int q = 0; q < f.nq; ++q) acc = cfma(acc, make_double2(f.w_re[w][q], f.w_im[w][q]), o[q]);
-  f.out[((size_t)(f.proj_base + w) * f.np + p) * f.lt + t] = acc;
+  for (int q = 0; q < 16; ++q) {
+    const double2 v = make_double2(o[q].x - sg * o[16 + q].y, o[q].y + sg * o[16 + q].x); // C + sg i S
+    acc = cfma(acc, make_double2(f.w_re[w][q], f.w_im[w][q]), v);
+  }
+  f.out[((size_t)(f.proj_base + w) * f.nmom + p) * f.lt + t] = acc;
 }
 
 void launch_project_apply(const ApplyArgs &f, hipStream_t stream)
 {
-  const size_t total = (size_t)f.nproj * f.np * f.lt;
+  const size_t total = (size_t)f.nproj * f.nmom * f.lt;
   hipLaunchKernelGGL(project_apply_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, f);
 }
 
