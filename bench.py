@@ -312,6 +312,38 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle, fi
     return res
 
 
+def abi_entry_points(torch, pc, u, d, u2, V, steps, warmup):
+    """Every drop-in entry point once per step on the 24^3x64 FP64 propagators:
+    ms per call and HBM GB/s of its algorithmic bytes (inputs read once,
+    outputs written once)."""
+    out = {}
+    correl = torch.empty(V, dtype=torch.complex128, device=dev_of(u))
+    c16 = [torch.empty(V, dtype=torch.complex128, device=dev_of(u)) for _ in range(16)]
+    scratch = torch.empty_like(u)
+    proj = pc.gamma_matrix(8)
+    cases = {
+        "meson_two_point": (lambda: pc.meson_two_point(correl, u, d, V, 5, 5), 2 * 2304 + 16),
+        "meson_all_sink_two_point": (lambda: pc.meson_all_sink_two_point(c16, u, d, V, 5), 2 * 2304 + 256),
+        "baryon_diquark": (lambda: pc.baryon_diquark(scratch, u, d, V, 5, 5), 3 * 2304),
+        "baryon_two_point IK_JL_NM": (lambda: pc.baryon_two_point(correl, u, d, u2, 0, V, 5, 5, 8), 3 * 2304 + 16),
+        "baryon_two_point IM_JL_NK": (lambda: pc.baryon_two_point(correl, u, d, u2, 5, V, 5, 5, 8), 3 * 2304 + 16),
+        "baryon_general_two_point IK_JL_NM": (
+            lambda: pc.baryon_general_two_point(correl, u, d, u2, 0, V, 5, 5, proj), 3 * 2304 + 16),
+        "baryon_sequential_two_point IK_JL_NM / N": (
+            lambda: pc.baryon_sequential_two_point(u, d, scratch, 0, 2, V, 5, 5, 8), 3 * 2304),
+    }
+    for name, (fn, bps) in cases.items():
+        wall, _ = time_steps(torch, fn, steps, warmup)
+        gbs = bps * V / (wall * 1e-3) / 1e9
+        out[name] = {"ms": round(wall, 4), "GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 3),
+                     "bytes_per_site": bps}
+    return out
+
+
+def dev_of(t):
+    return t.device
+
+
 def serving_leg(torch, args):
     """Serving loop: Poisson arrivals of mixed requests (50% meson grid, 30%
     proton two-point, 20% diquark) on lattices drawn from 16^3x32, 24^3x64,
@@ -409,6 +441,7 @@ def main():
         pc.baryon_two_point(correl, u, d, u2, 5, V, GAMMA_CG5, GAMMA_CG5, 8)
 
     wall_abi, _ = time_steps(torch, step_abi, args.steps, args.warmup)
+    abi_all = abi_entry_points(torch, pc, u, d, u2, V, max(5, args.steps // 5), args.warmup)
 
     del u_f, d_f, u2_f
     precision = None
@@ -481,6 +514,7 @@ def main():
             "sites_per_s": V / (wall_abi * 1e-3),
             "achieved_GBs": (BYTES_PER_SITE_FP64 + 16) * V / (wall_abi * 1e-3) / 1e9,
             "note": "one type (IM_JL_NK), one insertion, per-site output, like one reference call",
+            "entry_points": abi_all,
         },
         "precision_study": precision,
         "meson_grid_batch": grid,
