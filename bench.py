@@ -171,6 +171,67 @@ def cpu_baseline(torch, props, gpu_fn, target_s=10.0):
     }
 
 
+def proton_32c64(torch, pc, dev, steps, warmup, hip, timer):
+    """The proton step of the headline workload (Cg5, direct + exchange, 16
+    insertions, 27 momenta) on a 32^3x64 lattice, FP64 and FP32 storage."""
+    dims = (32, 32, 32, 64)
+    V = 32 ** 3 * 64
+    props = make_propagators(torch, dims, (4001, 4002, 4003), ALPHA, dev)
+    ws = torch.empty(pc.project_workspace_bytes(dims, 27), dtype=torch.uint8, device=dev)
+    out = torch.empty((16, 27, 64), dtype=torch.complex128, device=dev)
+    res = {"workload": "proton 2pt 32^3x64, Cg5, direct+exchange, 16 insertions, 27 momenta", "sites": V}
+    for name, ps, bps in (("fp64", props, BYTES_PER_SITE_FP64), ("fp32", None, BYTES_PER_SITE_FP32)):
+        if ps is None:
+            ps = [p.to(torch.complex64) for p in props]
+            del props[:]
+
+        def step():
+            pc.baryonTwoPointProject(*ps, TERMS, GAMMA_CG5, GAMMA_CG5, list(range(16)), MOM27, dims, out=out,
+                                     workspace=ws)
+
+        wall, kern = time_steps(torch, step, steps, warmup, hip, timer)
+        gbs = bps * V / (kern * 1e-3) / 1e9
+        res[name] = {"ms_per_step": wall, "kernel_ms": kern, "sites_per_s": V / (wall * 1e-3),
+                     "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        del ps
+    res["fp32_speedup"] = res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"]
+    return res
+
+
+def baryon_batch(torch, pc, dev, steps, warmup, hip=None, timer=None, nreq=16, dims=(16, 16, 16, 32)):
+    """Batching leg: the proton two-point (16 insertions, 27 momenta) for
+    `nreq` source positions on a 16^3x32 lattice -- each with its own u, d, u'
+    propagators -- as one batched launch versus `nreq` single calls."""
+    V = dims[0] * dims[1] * dims[2] * dims[3]
+    reqs = []
+    for n in range(nreq):
+        ps = make_propagators(torch, dims, (7000 + 3 * n, 7001 + 3 * n, 7002 + 3 * n), ALPHA, dev)
+        reqs.append((*ps, dims))
+    gammas = list(range(16))
+    outs = [torch.empty((16, 27, dims[3]), dtype=torch.complex128, device=dev) for _ in range(nreq)]
+    ws = torch.empty(pc.baryon_batch_workspace_bytes(reqs, 27), dtype=torch.uint8, device=dev)
+
+    def batched():
+        pc.baryonTwoPointProjectBatch(reqs, TERMS, GAMMA_CG5, GAMMA_CG5, gammas, MOM27, outs=outs, workspace=ws)
+
+    def singles():
+        for (pi, pj, pn, d), o in zip(reqs, outs):
+            pc.baryonTwoPointProject(pi, pj, pn, TERMS, GAMMA_CG5, GAMMA_CG5, gammas, MOM27, d, out=o, workspace=ws)
+
+    wall_b, kern_b = time_steps(torch, batched, steps, warmup, hip, timer)
+    got = [o.cpu() for o in outs]
+    wall_s, _ = time_steps(torch, singles, steps, warmup)
+    same = all(torch.equal(a, o.cpu()) for a, o in zip(got, outs))
+    gbs = BYTES_PER_SITE_FP64 * V * nreq / (wall_b * 1e-3) / 1e9
+    return {"workload": f"proton 2pt for {nreq} source positions, {dims[0]}^3x{dims[3]} each, Cg5, 16 insertions, "
+                        "27 momenta",
+            "requests": nreq, "batched_ms": wall_b, "batched_kernel_ms": kern_b,
+            "kernel_GBs": None if kern_b is None else BYTES_PER_SITE_FP64 * V * nreq / (kern_b * 1e-3) / 1e9,
+            "singles_ms": wall_s, "speedup": wall_s / wall_b,
+            "sites_per_s": V * nreq / (wall_b * 1e-3), "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+            "bit_identical_to_singles": same}
+
+
 def precision_study(torch, pc, dev, steps, warmup, hip, timer):
     """Sequential-source precision study: 32^3x64, propagators decaying as
     exp(-0.8|t|) (~20 decades over the lattice), stored in FP32.  Two forward
@@ -367,6 +428,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the 32^3x64 FP32 precision study")
     ap.add_argument("--no-grid", action="store_true", help="skip the batched meson grid leg")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batched-baryon and 32^3x64 proton legs")
     ap.add_argument("--only-grid", action="store_true", help="run only the batched meson grid leg")
     ap.add_argument("--grid-tuning", default=None,
                     help="sites_per_item,tile_sites,blocks_per_cu for the grid leg instead of autotuning")
@@ -393,6 +456,11 @@ def main():
 
     if args.only_serving:
         print(json.dumps({"serving": serving_leg(torch, args)}))
+        return
+
+    if args.only_batch:
+        print(json.dumps({"baryon_batch": baryon_batch(torch, pc, dev, args.steps, args.warmup, hip, timer),
+                          "proton_32c64": proton_32c64(torch, pc, dev, args.steps, args.warmup, hip, timer)}))
         return
 
     if args.only_grid:
@@ -448,6 +516,8 @@ def main():
     if not args.no_precision:
         del ws
         precision = precision_study(torch, pc, dev, args.steps, args.warmup, hip, timer)
+    big = proton_32c64(torch, pc, dev, args.steps, args.warmup, hip, timer)
+    batch = baryon_batch(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer)
 
     grid = None
     if not args.no_grid:
@@ -517,6 +587,8 @@ def main():
             "entry_points": abi_all,
         },
         "precision_study": precision,
+        "proton_32c64": big,
+        "baryon_batch": batch,
         "meson_grid_batch": grid,
         "serving": serving,
         "cpu_baseline": cpu,

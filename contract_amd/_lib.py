@@ -33,6 +33,8 @@ EXT_SYMBOLS = (
     "contract_project_workspace_bytes",
     "contract_baryon_twopoint_project",
     "contract_baryon_twopoint_project_sub",
+    "contract_baryon_batch_workspace_bytes",
+    "contract_baryon_twopoint_project_batch",
     "contract_set_kernel_timer",
     "contract_set_stream",
     "contract_meson_grid_workspace_bytes",
@@ -81,6 +83,10 @@ def lib() -> ctypes.CDLL:
     L.contract_baryon_twopoint_project.restype = I
     L.contract_baryon_twopoint_project_sub.argtypes = [P, P, P, P, I, P, P, I, I, I, P, I, P, I, P, P, S, P]
     L.contract_baryon_twopoint_project_sub.restype = I
+    L.contract_baryon_batch_workspace_bytes.argtypes = [P, I, I]
+    L.contract_baryon_batch_workspace_bytes.restype = S
+    L.contract_baryon_twopoint_project_batch.argtypes = [P, I, I, P, P, I, I, I, P, I, P, I, P, S, P]
+    L.contract_baryon_twopoint_project_batch.restype = I
     L.contract_set_kernel_timer.argtypes = [P, P]
     L.contract_set_kernel_timer.restype = I
     L.contract_set_stream.argtypes = [P]

@@ -23,6 +23,8 @@
 // Lane q of a 16-lane site group holds D[x][y] for q = 4x + y, and R[n][m]
 // for q = 4m + n.
 
+#include <atomic>
+
 #include "baryon.hpp"
 #include "device.hpp"
 #include "runtime.hpp"
@@ -306,7 +308,7 @@ template <int NT> struct PhaseWalk {
   //   m = p_x | p_y << 9 | p_z << 18 (p_mu mod G_mu; 0 for rows past np)
   int k[NT], m[NT];
 
-  __device__ __forceinline__ void init(const ProjectArgs &pa, size_t first, int g, int q)
+  __device__ __forceinline__ void init(const ProjectArgs &args, const ProjectReq &pa, size_t first, int g, int q)
   {
     const size_t slice = first / pa.slice_sites;
     const unsigned r0 = (unsigned)(first - slice * pa.slice_sites) + (unsigned)g;
@@ -323,10 +325,10 @@ template <int NT> struct PhaseWalk {
 #pragma unroll
     for (int tt = 0; tt < NT; ++tt) {
       const int p = tt * 16 + q;
-      const bool live = p < pa.np;
-      const int mx = live ? nonneg_mod(pa.mom[p][0], pa.gdims[0]) : 0;
-      const int my = live ? nonneg_mod(pa.mom[p][1], pa.gdims[1]) : 0;
-      const int mz = live ? nonneg_mod(pa.mom[p][2], pa.gdims[2]) : 0;
+      const bool live = p < args.np;
+      const int mx = live ? nonneg_mod(args.mom[p][0], pa.gdims[0]) : 0;
+      const int my = live ? nonneg_mod(args.mom[p][1], pa.gdims[1]) : 0;
+      const int mz = live ? nonneg_mod(args.mom[p][2], pa.gdims[2]) : 0;
       const int kx = (mx * ((xs * xr + pa.origin[0]) % pa.gdims[0])) % pa.gdims[0];
       const int ky = (my * ((y + pa.origin[1]) % pa.gdims[1])) % pa.gdims[1];
       const int kz = (mz * ((z + pa.origin[2]) % pa.gdims[2])) % pa.gdims[2];
@@ -335,7 +337,7 @@ template <int NT> struct PhaseWalk {
     }
   }
 
-  __device__ __forceinline__ double2 phase(const ProjectArgs &pa, const double2 *tab, int tt) const
+  __device__ __forceinline__ double2 phase(const ProjectReq &pa, const double2 *tab, int tt) const
   {
     int kx = k[tt] & 511;
     if (pa.order && ((pbase + y + z) & 1)) {
@@ -346,7 +348,7 @@ template <int NT> struct PhaseWalk {
                 tab[pa.gdims[0] + pa.gdims[1] + (k[tt] >> 18)]);
   }
 
-  __device__ __forceinline__ void advance(const ProjectArgs &pa)
+  __device__ __forceinline__ void advance(const ProjectReq &pa)
   {
     const int xs = pa.order ? 2 : 1;
     const int Gx = pa.gdims[0], Gy = pa.gdims[1], Gz = pa.gdims[2];
@@ -388,14 +390,14 @@ template <int NT> struct PhaseWalk {
 // S = -sum sin(k.x) R in four accumulators (C re, C im, S re, S im); the finish
 // forms P(+-k) = C +- iS.
 template <int NT>
-__device__ __forceinline__ void project_accumulate(dbl4 (&acc)[NT][4], const ProjectArgs &pa, const double2 *tab,
-                                                   double2 R, const PhaseWalk<NT> &walk)
+__device__ __forceinline__ void project_accumulate(dbl4 (&acc)[NT][4], int np, const ProjectReq &pa,
+                                                   const double2 *tab, double2 R, const PhaseWalk<NT> &walk)
 {
   double pr[NT], pi[NT];
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
     const double2 ph = walk.phase(pa, tab, tt); // (cos, -sin)
-    const bool live = tt * 16 + (int)(threadIdx.x & 15) < pa.np; // rows past np stay zero
+    const bool live = tt * 16 + (int)(threadIdx.x & 15) < np; // rows past np stay zero
     pr[tt] = live ? ph.x : 0.0;
     pi[tt] = live ? ph.y : 0.0;
   }
@@ -411,7 +413,7 @@ __device__ __forceinline__ void project_accumulate(dbl4 (&acc)[NT][4], const Pro
 // Build the phase tables exp(-2 pi i k / G_mu) of the three spatial directions
 // (global extents) in LDS ([Gx | Gy | Gz] entries); a few sincospi per lane per
 // work item.
-__device__ __forceinline__ void load_phase_tab(double2 *tab, const ProjectArgs &pa, int lane)
+__device__ __forceinline__ void load_phase_tab(double2 *tab, const ProjectReq &pa, int lane)
 {
   const int G0 = pa.gdims[0], G1 = pa.gdims[1], G2 = pa.gdims[2];
   const int n = G0 + G1 + G2;
@@ -425,23 +427,33 @@ __device__ __forceinline__ void load_phase_tab(double2 *tab, const ProjectArgs &
 }
 
 template <int NT>
-__device__ __forceinline__ void project_store(const dbl4 (&acc)[NT][4], const ProjectArgs &pa, size_t item, int lane)
+__device__ __forceinline__ void project_store(const dbl4 (&acc)[NT][4], int np, const ProjectReq &pa, size_t item,
+                                              int lane)
 {
   const int g = lane >> 4, q = lane & 15;
-  double2 *dst = pa.partial + item * (size_t)pa.np * 32;
+  double2 *dst = pa.partial + item * (size_t)np * 32;
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int p = tt * 16 + g + 4 * r; // v_mfma_f64_16x16x4f64: lane (g, q) register r holds D[g + 4r][q]
-      if (p < pa.np) {
+      if (p < np) {
         dst[p * 32 + q] = make_double2(acc[tt][0][r], acc[tt][1][r]);      // C
         dst[p * 32 + 16 + q] = make_double2(acc[tt][2][r], acc[tt][3][r]); // S
       }
     }
 }
 
-__device__ __forceinline__ void item_range(const ProjectArgs &pa, size_t item, size_t &first, size_t &last)
+// A wave-uniform value computed on the vector ALU (the 64-bit divisions of
+// item_range) moved to scalar registers.
+__device__ __forceinline__ size_t uniform_size(size_t v)
+{
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return (size_t)hi << 32 | lo;
+}
+
+__device__ __forceinline__ void item_range(const ProjectReq &pa, size_t item, size_t &first, size_t &last)
 {
   const size_t slice = item / pa.chunks_per_slice;
   const size_t c = item - slice * pa.chunks_per_slice;
@@ -449,8 +461,8 @@ __device__ __forceinline__ void item_range(const ProjectArgs &pa, size_t item, s
   first = s0 + c * (size_t)pa.chunk;
   last = first + pa.chunk;
   const size_t s1 = s0 + pa.slice_sites;
-  last = last < s1 ? last : s1;
-  first = first < s1 ? first : s1;
+  last = uniform_size(last < s1 ? last : s1);
+  first = uniform_size(first < s1 ? first : s1);
 }
 
 // FP32 storage: rescale a staged site of a propagator by an exact power of
@@ -481,110 +493,194 @@ __device__ __forceinline__ int normalise_site(float2 *site_lds, int q)
 #define CONTRACT_WPE32 2
 #endif
 
+// FP64 with up to two momentum tiles (one wave per SIMD) runs persistent
+// waves that prefetch across work items.  FP32 (two waves per SIMD) and the
+// larger FP64 tiles have no register room for the cross-item state: they run
+// one wave per item.
+#ifndef CONTRACT_PROJECT_PERSIST
+#define CONTRACT_PROJECT_PERSIST 1
+#endif
+template <typename Real, int NT> constexpr bool project_persistent()
+{
+  return CONTRACT_PROJECT_PERSIST && sizeof(Real) == 8 && NT <= 2;
+}
+
 template <int NT, typename Real, int GIJ = -1>
 __global__ __launch_bounds__(WAVE)
 __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : (NT == 1 ? CONTRACT_WPE32 : 1)))) void
 baryon_project_kernel(
-  BaryonProjectArgs a, ProjectArgs pa)
+  BaryonProjectArgs a, ProjectArgs args)
 {
   using C = cplx<Real>;
   constexpr int SL = site_lds<C>();
+  constexpr bool PERSIST = project_persistent<Real, NT>();
   __shared__ C lds[3][SITES_PER_WAVE * SL];
   extern __shared__ double2 tab[]; // Gx + Gy + Gz phase entries, sized at launch
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
-  const size_t item = blockIdx.x;
+  unsigned item = blockIdx.x; // global work item (uniform)
+  if (item >= args.total_items) return;
+  int r = 0; // its request
+  while (r + 1 < args.nreq && item >= args.req[r + 1].item0) ++r;
   size_t first, last;
-  item_range(pa, item, first, last);
-  load_phase_tab(tab, pa, lane);
+  item_range(args.req[r], item - args.req[r].item0, first, last);
+  load_phase_tab(tab, args.req[r], lane);
   const KlLane<C> kl(a.gkl, a.gklT, q);
-  PhaseWalk<NT> walk;
-  walk.init(pa, first, g, q);
 
-  dbl4 acc[NT][4];
-#pragma unroll
-  for (int tt = 0; tt < NT; ++tt)
-#pragma unroll
-    for (int u = 0; u < 4; ++u) acc[tt][u] = dbl4{0.0, 0.0, 0.0, 0.0};
-
-  const C *P[3] = {(const C *)a.propag_i, (const C *)a.propag_j, (const C *)a.propag_n};
-  auto nsites_of = [&](size_t s) {
-    const size_t left = last - s;
+  auto tile_sites = [](size_t s, size_t end) {
+    const size_t left = end - s;
     return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
   };
   Tile<C> buf[3];
-  if (first < last)
-#pragma unroll
-    for (int p = 0; p < 3; ++p) buf[p].load(P[p], first, nsites_of(first), lane);
+  {
+    const ProjectReq &pa = args.req[r];
+    const int n = tile_sites(first, last);
+    buf[0].load((const C *)pa.propag_i, first, n, lane);
+    buf[1].load((const C *)pa.propag_j, first, n, lane);
+    buf[2].load((const C *)pa.propag_n, first, n, lane);
+  }
 
-  for (size_t s = first; s < last; s += SITES_PER_WAVE) {
-#pragma unroll
-    for (int p = 0; p < 3; ++p) buf[p].store(lds[p], lane);
-    __syncthreads();
-    const int nsites = nsites_of(s);
-    const size_t next = s + SITES_PER_WAVE;
-    if (next < last)
-#pragma unroll
-      for (int p = 0; p < 3; ++p) buf[p].load(P[p], next, nsites_of(next), lane);
+  for (;;) {
+    const ProjectReq &pa = args.req[r];
+    // the wave's next item: its first tile is prefetched during this item's last step
+    const unsigned item_next = item + gridDim.x;
+    const bool more = PERSIST && item_next < args.total_items;
+    int r_next = r;
+    size_t first_next = 0, last_next = 0;
+    if (more) {
+      while (r_next + 1 < args.nreq && item_next >= args.req[r_next + 1].item0) ++r_next;
+      item_range(args.req[r_next], item_next - args.req[r_next].item0, first_next, last_next);
+    }
+    const C *Pi = (const C *)pa.propag_i, *Pj = (const C *)pa.propag_j, *Pn = (const C *)pa.propag_n;
+    const C *Qi = (const C *)args.req[r_next].propag_i, *Qj = (const C *)args.req[r_next].propag_j,
+            *Qn = (const C *)args.req[r_next].propag_n;
 
-    C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[2] + g * SL;
-    int ex = 0;
-    if constexpr (sizeof(Real) == 4) {
-      // P_i and P_j only: D of the rescaled blocks is O(1), so the tails'
-      // products with P_n are as large as P_n's own entries and cannot
-      // underflow unless a stored entry already is subnormal
-      ex = normalise_site(si, q) + normalise_site(sj, q);
+    PhaseWalk<NT> walk;
+    walk.init(args, pa, first, g, q);
+    dbl4 acc[NT][4];
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[tt][u] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    for (size_t s = first; s < last; s += SITES_PER_WAVE) {
+#pragma unroll
+      for (int p = 0; p < 3; ++p) buf[p].store(lds[p], lane);
+      __syncthreads();
+      const size_t next = s + SITES_PER_WAVE;
+      if (next < last) {
+        const int n = tile_sites(next, last);
+        buf[0].load(Pi, next, n, lane);
+        buf[1].load(Pj, next, n, lane);
+        buf[2].load(Pn, next, n, lane);
+      } else if (more) {
+        const int n = tile_sites(first_next, last_next);
+        buf[0].load(Qi, first_next, n, lane);
+        buf[1].load(Qj, first_next, n, lane);
+        buf[2].load(Qn, first_next, n, lane);
+      }
+
+      C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[2] + g * SL;
+      int ex = 0;
+      if constexpr (sizeof(Real) == 4) {
+        // P_i and P_j only: D of the rescaled blocks is O(1), so the tails'
+        // products with P_n are as large as P_n's own entries and cannot
+        // underflow unless a stored entry already is subnormal
+        ex = normalise_site(si, q) + normalise_site(sj, q);
+        __syncthreads();
+      }
+      C D[9];
+      if constexpr (GIJ >= 0)
+        diquark_tensor_static<GIJ>(D, si, sj, q);
+      else
+        diquark_tensor(D, si, sj, q, a.gij);
+      double2 R = make_double2(0.0, 0.0);
+      for (int t = 0; t < a.nterm; ++t) {
+        const C rt = baryon_tail_rt<C>(a.types[t], D, sn, si, lane, kl);
+        R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)rt.x, (double)rt.y));
+      }
+      if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
+      project_accumulate<NT>(acc, args.np, pa, tab, R, walk); // padded lanes carry R = 0
+      walk.advance(pa);
       __syncthreads();
     }
-    C D[9];
-    if constexpr (GIJ >= 0)
-      diquark_tensor_static<GIJ>(D, si, sj, q);
-    else
-      diquark_tensor(D, si, sj, q, a.gij);
-    double2 R = make_double2(0.0, 0.0);
-    for (int t = 0; t < a.nterm; ++t) {
-      const C r = baryon_tail_rt<C>(a.types[t], D, sn, si, lane, kl);
-      R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)r.x, (double)r.y));
+    project_store<NT>(acc, args.np, pa, item - pa.item0, lane);
+    if (!more) break;
+    if (r_next != r) {
+      load_phase_tab(tab, args.req[r_next], lane); // first read after the next step's barrier
     }
-    if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
-    project_accumulate<NT>(acc, pa, tab, R, walk); // padded lanes carry R = 0
-    walk.advance(pa);
-    __syncthreads();
+    item = item_next;
+    r = r_next;
+    first = first_next;
+    last = last_next;
   }
-  project_store<NT>(acc, pa, item, lane);
+}
+
+// Resident blocks per CU of a projection kernel, queried once per device
+// with the largest phase table.
+template <typename K> static unsigned resident_waves(K kernel)
+{
+  static std::atomic<int> cache[64];
+  int dev = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if ((per_cu = cache[dev].load(std::memory_order_relaxed)) <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, WAVE, sizeof(double2) * MAX_PHASE_TAB) !=
+          hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    cache[dev].store(per_cu, std::memory_order_relaxed);
+  }
+  return (unsigned)per_cu * (unsigned)device_cu_count();
+}
+
+template <int NT, typename Real, int GIJ>
+static void launch_project_nt(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t tab_bytes,
+                              hipStream_t stream)
+{
+  auto kernel = baryon_project_kernel<NT, Real, GIJ>;
+  unsigned grid = pa.total_items;
+  if (project_persistent<Real, NT>()) { // as many waves as the chip holds at once
+    const unsigned w = resident_waves(kernel);
+    grid = grid < w ? grid : w;
+  }
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(WAVE), tab_bytes, stream, a, pa);
 }
 
 template <typename Real, int GIJ>
-static void launch_project_g(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+static void launch_project_g(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
 {
   const int nt = (pa.np + 15) / 16;
-  const dim3 grid((unsigned)items), block(WAVE);
-  const size_t tab_bytes = sizeof(double2) * (pa.gdims[0] + pa.gdims[1] + pa.gdims[2]);
+  int tab = 0;
+  for (int r = 0; r < pa.nreq; ++r) {
+    const int n = pa.req[r].gdims[0] + pa.req[r].gdims[1] + pa.req[r].gdims[2];
+    tab = tab > n ? tab : n;
+  }
+  const size_t tab_bytes = sizeof(double2) * tab;
   switch (nt) {
-  case 1: hipLaunchKernelGGL((baryon_project_kernel<1, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
-  case 2: hipLaunchKernelGGL((baryon_project_kernel<2, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
-  case 3: hipLaunchKernelGGL((baryon_project_kernel<3, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
-  default: hipLaunchKernelGGL((baryon_project_kernel<4, Real, GIJ>), grid, block, tab_bytes, stream, a, pa); break;
+  case 1: launch_project_nt<1, Real, GIJ>(a, pa, tab_bytes, stream); break;
+  case 2: launch_project_nt<2, Real, GIJ>(a, pa, tab_bytes, stream); break;
+  case 3: launch_project_nt<3, Real, GIJ>(a, pa, tab_bytes, stream); break;
+  default: launch_project_nt<4, Real, GIJ>(a, pa, tab_bytes, stream); break;
   }
 }
 
 template <typename Real>
-static void launch_project_t(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+static void launch_project_t(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
 {
   if (a.gamma_ij == 5)
-    launch_project_g<Real, 5>(a, pa, items, stream);
+    launch_project_g<Real, 5>(a, pa, stream);
   else
-    launch_project_g<Real, -1>(a, pa, items, stream);
+    launch_project_g<Real, -1>(a, pa, stream);
 }
 
-void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
 {
-  launch_project_t<double>(a, pa, items, stream);
+  launch_project_t<double>(a, pa, stream);
 }
 
-void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream)
+void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
 {
-  launch_project_t<float>(a, pa, items, stream);
+  launch_project_t<float>(a, pa, stream);
 }
 
 } // namespace contract_amd

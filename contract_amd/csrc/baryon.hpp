@@ -34,55 +34,82 @@ struct BaryonSiteArgs {
 void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream);
 
 // ---- timeslice / momentum projection shared by the fused kernels --------
-struct ProjectArgs {
-  int dims[4]; // Lx, Ly, Lz, Lt
-  int gdims[3];  // global spatial extents (phases are exp(-2 pi i p (x + origin) / gdims))
-  int origin[3]; // global coordinates of local site 0
-  int order;   // 0 lexicographic [t][z][y][x]; 1 even-odd [parity][t][z][y][x/2]
-  int np;
-  int mom[MAX_MOM][3];
-  int chunk;   // sites per work item (multiple of 4)
-  int chunks_per_slice;
-  size_t slice_sites; // sites per (parity, t) slice
+constexpr int MAX_BATCH = 16; // requests (propagator triples) per fused launch
+
+// One request of a fused launch: its propagators, lattice geometry and work
+// decomposition.  Work item = a run of `chunk` sites inside one slice (a
+// timeslice, or one parity of it), so every item feeds exactly one timeslice.
+struct ProjectReq {
+  const void *propag_i, *propag_j, *propag_n;
   double2 *partial;   // [items][np slots][2][16]: C and S of each momentum slot
+  size_t slice_sites; // sites per (parity, t) slice
+  int dims[4];        // Lx, Ly, Lz, Lt
+  int gdims[3];       // global spatial extents (phases are exp(-2 pi i p (x + origin) / gdims))
+  int origin[3];      // global coordinates of local site 0
+  int order;          // 0 lexicographic [t][z][y][x]; 1 even-odd [parity][t][z][y][x/2]
+  int chunk;          // sites per work item (multiple of 4)
+  int chunks_per_slice;
+  unsigned item0;     // first work item of this request in the launch
+};
+
+// A launch runs one wave per resident slot; wave w takes work items w,
+// w + gridDim.x, ... over the concatenated items of all requests, prefetching
+// the next item's first tile while it finishes the current one.  Which wave
+// computes an item does not change the item's partial.
+struct ProjectArgs {
+  int np; // momentum slots (one per +-k pair)
+  int mom[MAX_MOM][3];
+  int nreq;
+  unsigned total_items; // over all requests
+  ProjectReq req[MAX_BATCH];
 };
 
 struct BaryonProjectArgs {
-  const void *propag_i, *propag_j, *propag_n;
   int gamma_ij; // selects a compile-time Gij form where one is instantiated
   SparseSpin gij, gkl, gklT;
   int nterm;
   int types[MAX_TERMS];
   double w_re[MAX_TERMS], w_im[MAX_TERMS];
 };
-void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream);
+void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream);
 // Same with complex64 propagators: FP32 per-site contraction on blocks rescaled
 // by exact powers of two, FP64 projection and sums.
-void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t items, hipStream_t stream);
+void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream);
 
 // partial[items][np][nq] -> open[t][np][nq]: per-timeslice sum over the
 // timeslice's work items (both parities), fixed order, Neumaier compensated.
-struct ReduceArgs {
+struct ReduceReq {
   const double2 *partial;
   double2 *open;
-  int np, lt, nq, order;
-  int chunks_per_slice;
+  int lt, order, chunks_per_slice;
+  int t0; // first timeslice of this request in the launch's grid
 };
-void launch_project_reduce(const ReduceArgs &r, hipStream_t stream);
+struct ReduceArgs {
+  int np, nq, nreq;
+  ReduceReq req[MAX_BATCH];
+};
+void launch_project_reduce(const ReduceArgs &r, int total_lt, hipStream_t stream);
 
 // open[t][slot][2][16] holds, per momentum slot k, C = sum cos(k.x) R and
 // S = -sum sin(k.x) R; requested momentum p is slot[p] with sign sgn[p]:
 // out[w][p][t] = sum_q W_w[q] (C[q] + sgn i S[q]), for up to
-// MAX_PROJ_PER_LAUNCH projectors per launch.
-struct ApplyArgs {
+// MAX_PROJ_PER_LAUNCH projectors and MAX_BATCH requests per launch.
+struct ApplyReq {
   const double2 *open;
-  int nslot, lt;
+  double2 *out;  // [nproj_total][nmom][lt]
+  int lt;
+  unsigned e0;   // first output element of this request in the launch
+};
+struct ApplyArgs {
+  int nslot;
   int nmom;      // requested momenta (outputs)
   int nproj;     // projectors in this launch
   int proj_base; // index of the first one in the output
   int slot[MAX_MOM], sgn[MAX_MOM];
   double w_re[MAX_PROJ_PER_LAUNCH][16], w_im[MAX_PROJ_PER_LAUNCH][16];
-  double2 *out;  // [nproj_total][nmom][lt]
+  int nreq;
+  unsigned total;
+  ApplyReq req[MAX_BATCH];
 };
 void launch_project_apply(const ApplyArgs &f, hipStream_t stream);
 

@@ -77,6 +77,8 @@ std::vector<MomSlot> pair_momenta(const int *mom, int nmom)
   return slots;
 }
 
+constexpr size_t MIN_ITEM_SITES = 128;
+
 ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item)
 {
   ProjectPlan p{};
@@ -93,6 +95,10 @@ ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_i
     if (cps < 1) cps = 1;
     chunk = (p.slice_sites + cps - 1) / cps;
   }
+  // at least 32 four-site steps per item: a shorter item spends a large part
+  // of its time on the prologue (phase tables, first tile) and the partial
+  // store (16^3x32: 4.48 -> 4.76 TB/s single, 4.82 -> 5.17 TB/s in a batch of 16)
+  if (sites_per_item <= 0 && chunk < MIN_ITEM_SITES) chunk = MIN_ITEM_SITES;
   chunk = (chunk + SITES_PER_WAVE - 1) / SITES_PER_WAVE * SITES_PER_WAVE;
   if (chunk < SITES_PER_WAVE) chunk = SITES_PER_WAVE;
   p.chunk = (int)chunk;
@@ -163,57 +169,43 @@ int resolve_geometry(Geometry &g, const int dims[4], int order, const int *globa
   return CONTRACT_OK;
 }
 
-void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry &geo, const int *momenta,
-                       const std::vector<MomSlot> &slots, void *ws)
-{
-  const int nmom = (int)slots.size();
-  std::memset(&pa, 0, sizeof(pa));
-  for (int mu = 0; mu < 4; ++mu) pa.dims[mu] = geo.dims[mu];
-  for (int mu = 0; mu < 3; ++mu) {
-    pa.gdims[mu] = geo.global[mu];
-    pa.origin[mu] = geo.origin[mu];
-  }
-  const int order = geo.order;
-  pa.order = order;
-  pa.np = nmom;
-  for (int p = 0; p < nmom; ++p)
-    for (int mu = 0; mu < 3; ++mu) pa.mom[p][mu] = momenta[3 * slots[p].rep + mu];
-  pa.chunk = plan.chunk;
-  pa.chunks_per_slice = plan.chunks_per_slice;
-  pa.slice_sites = plan.slice_sites;
-  pa.partial = reinterpret_cast<double2 *>(ws);
-}
-
 // Per-timeslice reduction of the work-item partials, then the projectors
 // (MAX_PROJ_PER_LAUNCH per launch), expanding each momentum slot to its +k
-// and -k outputs.
-void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const std::vector<MomSlot> &slots, int nmom,
-                       const zhost *W, int nproj, void *out, void *ws, hipStream_t stream)
+// and -k outputs; up to MAX_BATCH requests share each launch.
+void launch_finish_all(const FinishReq *reqs, int nreq, int nslot, const std::vector<MomSlot> &slots, int nmom,
+                       const zhost *W, int nproj, hipStream_t stream)
 {
   ReduceArgs r;
-  r.partial = pa.partial;
-  r.open = reinterpret_cast<double2 *>(static_cast<char *>(ws) + plan.partial_bytes);
-  r.np = pa.np;
-  r.lt = pa.dims[3];
+  std::memset(&r, 0, sizeof(r));
+  r.np = nslot;
   r.nq = 32;
-  r.order = pa.order;
-  r.chunks_per_slice = plan.chunks_per_slice;
-  launch_project_reduce(r, stream);
-  for (int base = 0; base < nproj; base += MAX_PROJ_PER_LAUNCH) {
-    ApplyArgs f;
-    std::memset(&f, 0, sizeof(f));
-    f.open = r.open;
-    f.nslot = pa.np;
-    f.lt = pa.dims[3];
-    f.nmom = nmom;
-    for (int k = 0; k < (int)slots.size(); ++k) {
-      f.slot[slots[k].plus] = k;
-      f.sgn[slots[k].plus] = 1;
-      if (slots[k].minus >= 0) {
-        f.slot[slots[k].minus] = k;
-        f.sgn[slots[k].minus] = -1;
-      }
+  r.nreq = nreq;
+  int total_lt = 0;
+  for (int k = 0; k < nreq; ++k) {
+    ReduceReq &q = r.req[k];
+    q.partial = reqs[k].partial;
+    q.open = reqs[k].open;
+    q.lt = reqs[k].lt;
+    q.order = reqs[k].order;
+    q.chunks_per_slice = reqs[k].chunks_per_slice;
+    q.t0 = total_lt;
+    total_lt += q.lt;
+  }
+  launch_project_reduce(r, total_lt, stream);
+  ApplyArgs f;
+  std::memset(&f, 0, sizeof(f));
+  f.nslot = nslot;
+  f.nmom = nmom;
+  for (int k = 0; k < (int)slots.size(); ++k) {
+    f.slot[slots[k].plus] = k;
+    f.sgn[slots[k].plus] = 1;
+    if (slots[k].minus >= 0) {
+      f.slot[slots[k].minus] = k;
+      f.sgn[slots[k].minus] = -1;
     }
+  }
+  f.nreq = nreq;
+  for (int base = 0; base < nproj; base += MAX_PROJ_PER_LAUNCH) {
     f.nproj = nproj - base < MAX_PROJ_PER_LAUNCH ? nproj - base : MAX_PROJ_PER_LAUNCH;
     f.proj_base = base;
     for (int w = 0; w < f.nproj; ++w)
@@ -221,7 +213,15 @@ void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const std
         f.w_re[w][q] = W[(base + w) * 16 + q].real();
         f.w_im[w][q] = W[(base + w) * 16 + q].imag();
       }
-    f.out = static_cast<double2 *>(out);
+    unsigned e = 0;
+    for (int k = 0; k < nreq; ++k) {
+      f.req[k].open = reqs[k].open;
+      f.req[k].out = static_cast<double2 *>(reqs[k].out);
+      f.req[k].lt = reqs[k].lt;
+      f.req[k].e0 = e;
+      e += (unsigned)(f.nproj * nmom * reqs[k].lt);
+    }
+    f.total = e;
     launch_project_apply(f, stream);
   }
 }
@@ -293,25 +293,58 @@ int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const 
                                          const double _Complex *weights, int nterm, int gamma_ij, int gamma_kl,
                                          const double _Complex *projectors, int nproj, const int *momenta, int nmom,
                                          const contract_sublattice *lattice, void *workspace, size_t workspace_bytes,
-                                         void *stream_)
+                                         void *stream)
 {
-  hipStream_t stream = static_cast<hipStream_t>(stream_);
-  int rc;
   if (!lattice) {
     set_last_error("lattice is NULL");
     return CONTRACT_ERR_ARG;
   }
-  Geometry geo;
-  if ((rc = resolve_geometry(geo, lattice->dims, lattice->site_order, lattice->global, lattice->origin)) !=
-      CONTRACT_OK)
-    return rc;
-  const int *dims = geo.dims;
-  const int site_order = geo.order;
-  if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
-  if (!out || !propag_i || !propag_j || !propag_n) {
-    set_last_error("NULL output or propagator pointer");
+  contract_baryon_request req;
+  req.propag_i = propag_i;
+  req.propag_j = propag_j;
+  req.propag_n = propag_n;
+  req.out = out;
+  req.lattice = *lattice;
+  return contract_baryon_twopoint_project_batch(&req, 1, precision, contract_types, weights, nterm, gamma_ij,
+                                                gamma_kl, projectors, nproj, momenta, nmom, workspace,
+                                                workspace_bytes, stream);
+}
+
+size_t contract_baryon_batch_workspace_bytes(const contract_baryon_request *reqs, int nreq, int nmom)
+{
+  if (!reqs || nreq < 1 || nmom < 1 || nmom > MAX_MOM) return 0;
+  size_t total = 0;
+  for (int r = 0; r < nreq; ++r) {
+    Geometry geo;
+    const contract_sublattice &L = reqs[r].lattice;
+    if (resolve_geometry(geo, L.dims, L.site_order, L.global, L.origin) != CONTRACT_OK) return 0;
+    total += plan_project(geo.dims, geo.order, nmom, 0).total;
+  }
+  return total;
+}
+
+int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, int nreq, int precision,
+                                           const int *contract_types, const double _Complex *weights, int nterm,
+                                           int gamma_ij, int gamma_kl, const double _Complex *projectors, int nproj,
+                                           const int *momenta, int nmom, void *workspace, size_t workspace_bytes,
+                                           void *stream_)
+{
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  int rc;
+  if (!reqs || nreq < 1) {
+    set_last_error("nreq = %d must be >= 1 with a non-NULL request array", nreq);
     return CONTRACT_ERR_ARG;
   }
+  std::vector<Geometry> geo(nreq);
+  for (int r = 0; r < nreq; ++r) {
+    const contract_sublattice &L = reqs[r].lattice;
+    if ((rc = resolve_geometry(geo[r], L.dims, L.site_order, L.global, L.origin)) != CONTRACT_OK) return rc;
+    if (!reqs[r].out || !reqs[r].propag_i || !reqs[r].propag_j || !reqs[r].propag_n) {
+      set_last_error("request %d: NULL output or propagator pointer", r);
+      return CONTRACT_ERR_ARG;
+    }
+  }
+  if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
   if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32) {
     set_last_error("precision %d is not CONTRACT_FP64 or CONTRACT_FP32", precision);
     return CONTRACT_ERR_ARG;
@@ -334,28 +367,33 @@ int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const 
     return CONTRACT_ERR_ARG;
   }
 
+  // per request: its work decomposition and workspace slice (partials, then
+  // per-timeslice sums); +k/-k momenta share a slot
   const std::vector<MomSlot> slots = pair_momenta(momenta, nmom);
-  const ProjectPlan plan = plan_project(dims, site_order, (int)slots.size(), 0);
+  const int nslot = (int)slots.size();
+  std::vector<ProjectPlan> plan(nreq);
+  std::vector<size_t> offset(nreq);
+  size_t total = 0;
+  for (int r = 0; r < nreq; ++r) {
+    plan[r] = plan_project(geo[r].dims, geo[r].order, nslot, 0);
+    offset[r] = total;
+    total += plan[r].total;
+  }
   void *ws = workspace;
   bool own = false;
   if (ws) {
-    if (workspace_bytes < plan.total) {
-      set_last_error("workspace of %zu bytes is smaller than the %zu needed", workspace_bytes, plan.total);
+    if (workspace_bytes < total) {
+      set_last_error("workspace of %zu bytes is smaller than the %zu needed", workspace_bytes, total);
       return CONTRACT_ERR_WORKSPACE;
     }
   } else {
-    CONTRACT_EXT_HIP(hipMallocAsync(&ws, plan.total, stream));
+    CONTRACT_EXT_HIP(hipMallocAsync(&ws, total, stream));
     own = true;
   }
-
-  ProjectArgs pa;
-  fill_project_args(pa, plan, geo, momenta, slots, ws);
+  char *wsb = static_cast<char *>(ws);
 
   BaryonProjectArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.propag_i = propag_i;
-  a.propag_j = propag_j;
-  a.propag_n = propag_n;
   a.gamma_ij = gamma_ij;
   a.gij = sparse_of(gamma_matrix(gamma_ij));
   a.gkl = sparse_of(gamma_matrix(gamma_kl));
@@ -367,18 +405,66 @@ int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const 
     a.w_re[t] = wt[2 * t];
     a.w_im[t] = wt[2 * t + 1];
   }
+
+  std::vector<ProjectReq> preq(nreq);
+  for (int r = 0; r < nreq; ++r) {
+    ProjectReq &R = preq[r];
+    std::memset(&R, 0, sizeof(R));
+    R.propag_i = reqs[r].propag_i;
+    R.propag_j = reqs[r].propag_j;
+    R.propag_n = reqs[r].propag_n;
+    R.partial = reinterpret_cast<double2 *>(wsb + offset[r]);
+    R.slice_sites = plan[r].slice_sites;
+    for (int mu = 0; mu < 4; ++mu) R.dims[mu] = geo[r].dims[mu];
+    for (int mu = 0; mu < 3; ++mu) {
+      R.gdims[mu] = geo[r].global[mu];
+      R.origin[mu] = geo[r].origin[mu];
+    }
+    R.order = geo[r].order;
+    R.chunk = plan[r].chunk;
+    R.chunks_per_slice = plan[r].chunks_per_slice;
+  }
+
   CONTRACT_EXT_HIP(timer_begin(stream));
-  if (precision == CONTRACT_FP64)
-    launch_baryon_project(a, pa, plan.items, stream);
-  else
-    launch_baryon_project_fp32(a, pa, plan.items, stream);
-  CONTRACT_EXT_HIP(hipGetLastError());
+  for (int r0 = 0; r0 < nreq; r0 += MAX_BATCH) {
+    ProjectArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.np = nslot;
+    for (int k = 0; k < nslot; ++k)
+      for (int mu = 0; mu < 3; ++mu) pa.mom[k][mu] = momenta[3 * slots[k].rep + mu];
+    pa.nreq = std::min(nreq - r0, MAX_BATCH);
+    size_t items = 0;
+    for (int r = 0; r < pa.nreq; ++r) {
+      pa.req[r] = preq[r0 + r];
+      pa.req[r].item0 = (unsigned)items;
+      items += plan[r0 + r].items;
+    }
+    pa.total_items = (unsigned)items;
+    if (precision == CONTRACT_FP64)
+      launch_baryon_project(a, pa, stream);
+    else
+      launch_baryon_project_fp32(a, pa, stream);
+    CONTRACT_EXT_HIP(hipGetLastError());
+  }
   CONTRACT_EXT_HIP(timer_end(stream));
 
   std::vector<zhost> W((size_t)nproj * 16);
   const double *pm = reinterpret_cast<const double *>(projectors);
   for (size_t e = 0; e < W.size(); ++e) W[e] = zhost(pm[2 * e], pm[2 * e + 1]);
-  launch_finish_all(pa, plan, slots, nmom, W.data(), nproj, out, ws, stream);
+  for (int r0 = 0; r0 < nreq; r0 += MAX_BATCH) {
+    FinishReq fr[MAX_BATCH];
+    const int n = std::min(nreq - r0, MAX_BATCH);
+    for (int k = 0; k < n; ++k) {
+      const int r = r0 + k;
+      fr[k].partial = preq[r].partial;
+      fr[k].open = reinterpret_cast<double2 *>(wsb + offset[r] + plan[r].partial_bytes);
+      fr[k].out = reqs[r].out;
+      fr[k].lt = geo[r].dims[3];
+      fr[k].order = geo[r].order;
+      fr[k].chunks_per_slice = plan[r].chunks_per_slice;
+    }
+    launch_finish_all(fr, n, nslot, slots, nmom, W.data(), nproj, stream);
+  }
   CONTRACT_EXT_HIP(hipGetLastError());
   if (own) CONTRACT_EXT_HIP(hipFreeAsync(ws, stream));
   return CONTRACT_OK;

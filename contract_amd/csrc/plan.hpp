@@ -45,9 +45,16 @@ struct Geometry {
   int dims[4], global[3], origin[3], order;
 };
 int resolve_geometry(Geometry &g, const int dims[4], int order, const int *global, const int *origin);
-void fill_project_args(ProjectArgs &pa, const ProjectPlan &plan, const Geometry &geo, const int *momenta,
-                       const std::vector<MomSlot> &slots, void *ws);
-void launch_finish_all(const ProjectArgs &pa, const ProjectPlan &plan, const std::vector<MomSlot> &slots, int nmom,
-                       const zhost *W, int nproj, void *out, void *ws, hipStream_t stream);
+// Per-timeslice sums of each request's work-item partials into its `open`,
+// then the projectors, each momentum slot expanded to its +k and -k outputs
+// (nreq <= MAX_BATCH requests per call).
+struct FinishReq {
+  const double2 *partial;
+  double2 *open;
+  void *out;
+  int lt, order, chunks_per_slice;
+};
+void launch_finish_all(const FinishReq *reqs, int nreq, int nslot, const std::vector<MomSlot> &slots, int nmom,
+                       const zhost *W, int nproj, hipStream_t stream);
 
 } // namespace contract_amd

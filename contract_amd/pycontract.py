@@ -293,6 +293,74 @@ def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, ga
     return out
 
 
+class BaryonRequest(ctypes.Structure):
+    """contract_baryon_request."""
+    _fields_ = [("propag_i", ctypes.c_void_p), ("propag_j", ctypes.c_void_p), ("propag_n", ctypes.c_void_p),
+                ("out", ctypes.c_void_p), ("lattice", Sublattice)]
+
+
+def _baryon_requests(requests):
+    """Request tuples (Pi, Pj, Pn, dims[, site_order[, global_dims[, origin]]])
+    as a contract_baryon_request array."""
+    arr = (BaryonRequest * len(requests))()
+    for r, req in zip(arr, requests):
+        r.propag_i, r.propag_j, r.propag_n = _ptr(req[0]), _ptr(req[1]), _ptr(req[2])
+        for mu in range(4):
+            r.lattice.dims[mu] = int(req[3][mu])
+        r.lattice.site_order = int(req[4]) if len(req) > 4 else LEXICOGRAPHIC
+        gl = req[5] if len(req) > 5 and req[5] is not None else ()
+        og = req[6] if len(req) > 6 and req[6] is not None else ()
+        for mu in range(4):
+            r.lattice.global_[mu] = int(gl[mu]) if mu < len(gl) else 0
+            r.lattice.origin[mu] = int(og[mu]) if mu < len(og) else 0
+    return arr
+
+
+def baryon_batch_workspace_bytes(requests, nmom: int) -> int:
+    arr = _baryon_requests(requests)
+    return int(lib().contract_baryon_batch_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(arr), int(nmom)))
+
+
+def baryonTwoPointProjectBatch(requests, terms, gamma_ij: int, gamma_kl: int, insertions, momenta, outs=None,
+                               workspace=None, stream=None):
+    """Several fused baryon two-points in one launch.  ``requests`` holds
+    tuples (Pi, Pj, Pn, dims[, site_order[, global_dims[, origin]]]); terms,
+    gammas, insertions and momenta are shared.  Returns one
+    ``[len(insertions), len(momenta), Lt]`` complex128 tensor per request,
+    each equal to what baryonTwoPointProject gives for that request alone."""
+    import torch
+    if not requests:
+        return []
+    dtype = requests[0][0].dtype
+    precision = CONTRACT_FP32 if dtype == torch.complex64 else CONTRACT_FP64
+    for req in requests:
+        for p in req[:3]:
+            if p.dtype != dtype:
+                raise TypeError("all propagators of a batch must have the same dtype")
+            _volume(p)
+    W = _projectors(insertions)
+    mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
+    types = np.ascontiguousarray([int(t) for t, _ in terms], dtype=np.int32)
+    weights = np.ascontiguousarray([complex(w) for _, w in terms], dtype=np.complex128)
+    arr = _baryon_requests(requests)
+    if outs is None:
+        outs = [torch.empty((W.shape[0], mom.shape[0], int(req[3][3])), dtype=torch.complex128,
+                            device=req[0].device) for req in requests]
+    for r, o in zip(arr, outs):
+        r.out = _ptr(o)
+    ws_ptr, ws_bytes = 0, 0
+    if workspace is not None:
+        ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    vp = ctypes.c_void_p
+    rc = lib().contract_baryon_twopoint_project_batch(
+        ctypes.cast(arr, vp), len(arr), precision, types.ctypes.data_as(vp), weights.ctypes.data_as(vp),
+        len(types), int(gamma_ij), int(gamma_kl), W.ctypes.data_as(vp), W.shape[0], mom.ctypes.data_as(vp),
+        mom.shape[0], ws_ptr, ws_bytes, 0 if stream is None else int(stream))
+    if rc != 0:
+        raise RuntimeError(f"contract_baryon_twopoint_project_batch failed ({rc}): {last_error()}")
+    return outs
+
+
 # ---------------------------------------------------------------------------
 # Batched meson Gamma x Gamma' grids (include/contract_ext.h)
 # ---------------------------------------------------------------------------

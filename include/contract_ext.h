@@ -87,6 +87,25 @@ int contract_baryon_twopoint_project_sub(void *out, const void *propag_i, const 
                                          const contract_sublattice *lattice, void *workspace, size_t workspace_bytes,
                                          void *stream);
 
+/* Several baryon two-points (e.g. source positions, flavour combinations or
+ * lattices of different size) in one fused launch: every request has its
+ * own propagators, output out[nproj][nmom][Lt] and lattice; terms, gammas,
+ * projectors, momenta and precision are shared.  Up to 16 requests share a
+ * launch.  A request's result does not depend on the rest of the batch. */
+typedef struct {
+  const void *propag_i, *propag_j, *propag_n;
+  void *out;
+  contract_sublattice lattice;
+} contract_baryon_request;
+
+size_t contract_baryon_batch_workspace_bytes(const contract_baryon_request *reqs, int nreq, int nmom);
+
+int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, int nreq, int precision,
+                                           const int *contract_types, const double _Complex *weights, int nterm,
+                                           int gamma_ij, int gamma_kl, const double _Complex *projectors, int nproj,
+                                           const int *momenta, int nmom, void *workspace, size_t workspace_bytes,
+                                           void *stream);
+
 /* ---- batched meson Gamma x Gamma' grids -------------------------------
  * One request = two propagators on one lattice.  For every request the full
  * 16 x 16 grid of meson_two_point(propag_i, propag_j, volume, g1, g2) (the

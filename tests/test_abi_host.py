@@ -121,6 +121,44 @@ def test_extension_rejects_bad_arguments_with_codes():
     assert L.contract_project_workspace_bytes(dims.ctypes.data, 0, 0) == 0
 
 
+def test_baryon_batch_validation_and_workspace():
+    import ctypes
+    import numpy as np
+    import contract_amd
+    from contract_amd import pycontract as pc
+    L = contract_amd.lib()
+    vp = ctypes.c_void_p
+    mom = np.zeros((1, 3), np.int32)
+    types = np.array([0], np.int32)
+    w = np.array([1.0], np.complex128)
+    proj = np.eye(4, dtype=np.complex128)
+
+    def call(arr, n):
+        return L.contract_baryon_twopoint_project_batch(arr, n, 0, types.ctypes.data, w.ctypes.data, 1, 5, 5,
+                                                        proj.ctypes.data, 1, mom.ctypes.data, 1, None, 0, None)
+
+    assert call(None, 1) != 0 and "nreq" in contract_amd.last_error()
+    arr = (pc.BaryonRequest * 2)()
+    for r in arr:
+        r.propag_i = r.propag_j = r.propag_n = r.out = 1
+        r.lattice.dims[:] = [4, 4, 4, 8]
+    arr[1].propag_n = None
+    assert call(ctypes.cast(arr, vp), 2) != 0 and "request 1" in contract_amd.last_error()
+    arr[1].propag_n = 1
+    arr[1].lattice.site_order = 1
+    arr[1].lattice.dims[0] = 3
+    assert call(ctypes.cast(arr, vp), 2) != 0 and "even" in contract_amd.last_error()
+    # the batch's workspace is the sum of its requests' own
+    a = (pc.BaryonRequest * 2)()
+    a[0].lattice.dims[:] = [4, 4, 4, 8]
+    a[1].lattice.dims[:] = [6, 4, 2, 6]
+    a[1].lattice.site_order = 1
+    single = [L.contract_project_workspace_bytes(np.array(d, np.int32).ctypes.data, o, 27)
+              for d, o in (([4, 4, 4, 8], 0), ([6, 4, 2, 6], 1))]
+    assert L.contract_baryon_batch_workspace_bytes(ctypes.cast(a, vp), 2, 27) == sum(single) > 0
+    assert L.contract_baryon_batch_workspace_bytes(ctypes.cast(a, vp), 2, 0) == 0
+
+
 def test_meson_grid_rejects_bad_arguments_with_codes():
     import ctypes
     import numpy as np
