@@ -193,6 +193,15 @@ template <typename C> __device__ __forceinline__ C group16_reduce_scatter(C (&v)
 // site is padded by one 16-byte chunk so the four sites of a tile start on
 // different banks.  Sites past `nsites` (ragged tail) are zero-filled.
 // ---------------------------------------------------------------------------
+// Propagator streams are read once: non-temporal loads keep them out of the
+// caches' reuse set (+1-5% over plain loads in the fused baryon kernel).
+__device__ __forceinline__ uint4 stream_load(const uint4 *p)
+{
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+  return make_uint4(w.x, w.y, w.z, w.w);
+}
+
 template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
   static constexpr int SITE16 = SITE_ELEMS * (int)sizeof(C) / 16;     // 144 (FP64) / 72 (FP32)
   static constexpr int LDS16 = SITE16 + 1;                             // padded site stride
@@ -209,7 +218,7 @@ template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
     for (int it = 0; it < PER_LANE; ++it) {
       const int e = it * WAVE + lane;
       if (CHUNKS % WAVE == 0 || it + 1 < PER_LANE || e < CHUNKS) // skip lanes past the tile
-        v[it] = (e < total) ? base[e] : make_uint4(0, 0, 0, 0);
+        v[it] = (e < total) ? stream_load(base + e) : make_uint4(0, 0, 0, 0);
     }
   }
 

@@ -155,12 +155,13 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
       if (left >= (size_t)NS) {
 #pragma unroll
         for (int it = 0; it < PER_THREAD; ++it)
-          if (active(it)) buf[it] = (second[it] ? b1 : b0)[src_off[it]];
+          if (active(it)) buf[it] = stream_load((second[it] ? b1 : b0) + src_off[it]);
       } else {
         const int total = (int)left * T::SITE16;
 #pragma unroll
         for (int it = 0; it < PER_THREAD; ++it)
-          if (active(it)) buf[it] = src_off[it] < total ? (second[it] ? b1 : b0)[src_off[it]] : make_uint4(0, 0, 0, 0);
+          if (active(it))
+            buf[it] = src_off[it] < total ? stream_load((second[it] ? b1 : b0) + src_off[it]) : make_uint4(0, 0, 0, 0);
       }
     };
     auto store = [&]() {
