@@ -568,16 +568,13 @@ baryon_project_kernel(
       for (int p = 0; p < 3; ++p) buf[p].store(lds[p], lane);
       __syncthreads();
       const size_t next = s + SITES_PER_WAVE;
-      if (next < last) {
-        const int n = tile_sites(next, last);
-        buf[0].load(Pi, next, n, lane);
-        buf[1].load(Pj, next, n, lane);
-        buf[2].load(Pn, next, n, lane);
-      } else if (more) {
-        const int n = tile_sites(first_next, last_next);
-        buf[0].load(Qi, first_next, n, lane);
-        buf[1].load(Qj, first_next, n, lane);
-        buf[2].load(Qn, first_next, n, lane);
+      const bool here = next < last; // else the next item's first tile (uniform)
+      if (here || more) {
+        const size_t st = here ? next : first_next;
+        const int n = here ? tile_sites(next, last) : tile_sites(first_next, last_next);
+        buf[0].load(here ? Pi : Qi, st, n, lane);
+        buf[1].load(here ? Pj : Qj, st, n, lane);
+        buf[2].load(here ? Pn : Qn, st, n, lane);
       }
 
       C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[2] + g * SL;

@@ -193,8 +193,28 @@ template <typename C> __device__ __forceinline__ C group16_reduce_scatter(C (&v)
 // site is padded by one 16-byte chunk so the four sites of a tile start on
 // different banks.  Sites past `nsites` (ragged tail) are zero-filled.
 // ---------------------------------------------------------------------------
-// Propagator streams are read once: non-temporal loads keep them out of the
-// caches' reuse set (+1-5% over plain loads in the fused baryon kernel).
+// Tile loads go through a buffer resource whose range ends at the tile's last
+// valid byte: the hardware returns zero past it, so a ragged tile needs no lane
+// mask, bounds select or zero fill (a masked or selected load on one side of a
+// branch makes the compiler wait for it where the paths join).  The streams
+// are read once: the loads are non-temporal, which keeps them out of the
+// caches' reuse set (+1-5% over plain loads).
+constexpr int BUFFER_RSRC_WORD3 = 0x00020000; // gfx9 raw buffer: 32-bit data format
+constexpr int BUFFER_AUX_NT = 2;              // cache policy: non-temporal
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const void *base, unsigned bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, BUFFER_RSRC_WORD3);
+}
+
+__device__ __forceinline__ uint4 stream_load(__amdgpu_buffer_rsrc_t rsrc, int byte_offset)
+{
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rsrc, byte_offset, 0, BUFFER_AUX_NT);
+  return make_uint4(w.x, w.y, w.z, w.w);
+}
+
+// Plain non-temporal load of one 16-byte chunk.
 __device__ __forceinline__ uint4 stream_load(const uint4 *p)
 {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -210,16 +230,12 @@ template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
   static constexpr int PER_LANE = (CHUNKS + WAVE - 1) / WAVE;         // 4 sites: 9 / 5; 2 sites: 5 / 3
   uint4 v[PER_LANE];
 
+  // Chunks past nsites (ragged tile) and lanes past the tile read as zero.
   __device__ __forceinline__ void load(const C *__restrict__ src, size_t first_site, int nsites, int lane)
   {
-    const uint4 *base = reinterpret_cast<const uint4 *>(src + first_site * SITE_ELEMS);
-    const int total = nsites * SITE16;
+    const __amdgpu_buffer_rsrc_t rsrc = tile_rsrc(src + first_site * SITE_ELEMS, (unsigned)(nsites * SITE16 * 16));
 #pragma unroll
-    for (int it = 0; it < PER_LANE; ++it) {
-      const int e = it * WAVE + lane;
-      if (CHUNKS % WAVE == 0 || it + 1 < PER_LANE || e < CHUNKS) // skip lanes past the tile
-        v[it] = (e < total) ? stream_load(base + e) : make_uint4(0, 0, 0, 0);
-    }
+    for (int it = 0; it < PER_LANE; ++it) v[it] = stream_load(rsrc, (it * WAVE + lane) * 16);
   }
 
   __device__ __forceinline__ void store(C *__restrict__ lds, int lane) const
@@ -228,7 +244,7 @@ template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
 #pragma unroll
     for (int it = 0; it < PER_LANE; ++it) {
       const int e = it * WAVE + lane;
-      if (CHUNKS % WAVE == 0 || e < CHUNKS) {
+      if (CHUNKS % WAVE == 0 || it + 1 < PER_LANE || e < CHUNKS) { // lanes past the tile hold no chunk
         const int s = e / SITE16;
         l[s * LDS16 + (e - s * SITE16)] = v[it];
       }
@@ -262,7 +278,15 @@ __device__ __forceinline__ void unstage_sites(C *__restrict__ dst, const C *__re
     const int e = it * WAVE + lane;
     if (e < total) {
       const int s = e / T::SITE16;
+#ifdef CONTRACT_NT_STORES
+      {
+        typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+        const uint4 v = l[s * T::LDS16 + (e - s * T::SITE16)];
+        __builtin_nontemporal_store(u32x4{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4 *>(base + e));
+      }
+#else
       base[e] = l[s * T::LDS16 + (e - s * T::SITE16)];
+#endif
     }
   }
 }
