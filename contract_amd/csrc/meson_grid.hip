@@ -62,8 +62,7 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
   using T = Tile<C>;
   constexpr int SL = T::LDS_ELEMS;               // padded site stride (complex entries)
   constexpr int CH = NS * T::SITE16;             // 16-byte chunks per propagator tile
-  constexpr int NCH = 2 * CH;                    // both propagators
-  constexpr int PER_THREAD = (NCH + GRID_THREADS - 1) / GRID_THREADS;
+  constexpr int PER_PROP = (CH + GRID_THREADS - 1) / GRID_THREADS; // chunks per thread and propagator
   constexpr int TILE16 = NS * T::LDS16;          // one propagator tile in LDS, 16-byte units
   __shared__ C lds[2][NS * SL];                  // [propagator]
   extern __shared__ double2 tab[];
@@ -72,20 +71,17 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
   const int g = lane >> 4, q = lane & 15;
   const int i = q >> 2, j = q & 3;
 
-  // this thread's chunks of a tile: offset from the tile's first site in the
-  // propagator, which propagator, and the LDS slot (the same every step)
-  int src_off[PER_THREAD], dst_off[PER_THREAD];
-  bool second[PER_THREAD];
+  // this thread's chunks of a propagator tile, e = it * GRID_THREADS + tid
+  // (a wave's loads all come from one propagator, so each takes one buffer
+  // resource), and their LDS slots (the same every step)
+  int dst_off[PER_PROP];
 #pragma unroll
-  for (int it = 0; it < PER_THREAD; ++it) {
+  for (int it = 0; it < PER_PROP; ++it) {
     const int e = it * GRID_THREADS + tid;
-    second[it] = e >= CH;
-    const int ec = e - (second[it] ? CH : 0);
-    const int s = ec / T::SITE16;
-    src_off[it] = ec;
-    dst_off[it] = (second[it] ? TILE16 : 0) + s * T::LDS16 + (ec - s * T::SITE16);
+    const int s = e / T::SITE16;
+    dst_off[it] = s * T::LDS16 + (e - s * T::SITE16);
   }
-  auto active = [&](int it) { return NCH % GRID_THREADS == 0 || it + 1 < PER_THREAD || tid < NCH % GRID_THREADS; };
+  auto active = [&](int it) { return CH % GRID_THREADS == 0 || it + 1 < PER_PROP || tid < CH % GRID_THREADS; };
 
   for (size_t item = blockIdx.x; item < a.total_items; item += gridDim.x) {
     int r = 0; // request of this item (uniform over the workgroup)
@@ -147,28 +143,26 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
     // the workgroup stages NS sites of both propagators per step; the next
     // step's chunks are in flight in registers while this one is computed
     const C *P0 = static_cast<const C *>(R.propag_i), *P1 = static_cast<const C *>(R.propag_j);
-    uint4 buf[PER_THREAD];
-    auto load = [&](size_t st) {
-      const uint4 *b0 = reinterpret_cast<const uint4 *>(P0 + st * SITE_ELEMS);
-      const uint4 *b1 = reinterpret_cast<const uint4 *>(P1 + st * SITE_ELEMS);
+    uint4 buf[2][PER_PROP];
+    auto load = [&](size_t st) { // chunks past the item's end read as zero
       const size_t left = last - st;
-      if (left >= (size_t)NS) {
+      const unsigned bytes = (unsigned)((left < (size_t)NS ? left : (size_t)NS) * T::SITE16 * 16);
+      const __amdgpu_buffer_rsrc_t r0 = tile_rsrc(P0 + st * SITE_ELEMS, bytes);
+      const __amdgpu_buffer_rsrc_t r1 = tile_rsrc(P1 + st * SITE_ELEMS, bytes);
 #pragma unroll
-        for (int it = 0; it < PER_THREAD; ++it)
-          if (active(it)) buf[it] = stream_load((second[it] ? b1 : b0) + src_off[it]);
-      } else {
-        const int total = (int)left * T::SITE16;
-#pragma unroll
-        for (int it = 0; it < PER_THREAD; ++it)
-          if (active(it))
-            buf[it] = src_off[it] < total ? stream_load((second[it] ? b1 : b0) + src_off[it]) : make_uint4(0, 0, 0, 0);
+      for (int it = 0; it < PER_PROP; ++it) {
+        buf[0][it] = stream_load(r0, (it * GRID_THREADS + tid) * 16);
+        buf[1][it] = stream_load(r1, (it * GRID_THREADS + tid) * 16);
       }
     };
     auto store = [&]() {
       uint4 *l16 = reinterpret_cast<uint4 *>(&lds[0][0]);
 #pragma unroll
-      for (int it = 0; it < PER_THREAD; ++it)
-        if (active(it)) l16[dst_off[it]] = buf[it];
+      for (int it = 0; it < PER_PROP; ++it)
+        if (active(it)) {
+          l16[dst_off[it]] = buf[0][it];
+          l16[TILE16 + dst_off[it]] = buf[1][it];
+        }
     };
     load(first);
     for (size_t st = first; st < last; st += NS) {
