@@ -428,6 +428,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the 32^3x64 FP32 precision study")
     ap.add_argument("--no-grid", action="store_true", help="skip the batched meson grid leg")
+    ap.add_argument("--no-big", action="store_true", help="skip the 32^3x64 proton leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-baryon leg")
+    ap.add_argument("--only-big", action="store_true", help="run only the 32^3x64 proton leg")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched-baryon and 32^3x64 proton legs")
     ap.add_argument("--only-grid", action="store_true", help="run only the batched meson grid leg")
@@ -456,6 +459,10 @@ def main():
 
     if args.only_serving:
         print(json.dumps({"serving": serving_leg(torch, args)}))
+        return
+
+    if args.only_big:
+        print(json.dumps({"proton_32c64": proton_32c64(torch, pc, dev, args.steps, args.warmup, hip, timer)}))
         return
 
     if args.only_batch:
@@ -516,8 +523,8 @@ def main():
     if not args.no_precision:
         del ws
         precision = precision_study(torch, pc, dev, args.steps, args.warmup, hip, timer)
-    big = proton_32c64(torch, pc, dev, args.steps, args.warmup, hip, timer)
-    batch = baryon_batch(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer)
+    big = None if args.no_big else proton_32c64(torch, pc, dev, args.steps, args.warmup, hip, timer)
+    batch = None if args.no_batch else baryon_batch(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer)
 
     grid = None
     if not args.no_grid:
