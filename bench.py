@@ -180,7 +180,9 @@ def proton_32c64(torch, pc, dev, steps, warmup, hip, timer):
     ws = torch.empty(pc.project_workspace_bytes(dims, 27), dtype=torch.uint8, device=dev)
     out = torch.empty((16, 27, 64), dtype=torch.complex128, device=dev)
     res = {"workload": "proton 2pt 32^3x64, Cg5, direct+exchange, 16 insertions, 27 momenta", "sites": V}
-    for name, ps, bps in (("fp64", props, BYTES_PER_SITE_FP64), ("fp32", None, BYTES_PER_SITE_FP32)):
+    scaled = [pc.toScaledFP32(p) for p in props]
+    for name, ps, bps in (("fp64", props, BYTES_PER_SITE_FP64), ("fp32", None, BYTES_PER_SITE_FP32),
+                          ("fp32_scaled", scaled, BYTES_PER_SITE_FP32 + 3)):
         if ps is None:
             ps = [p.to(torch.complex64) for p in props]
             del props[:]
@@ -193,8 +195,14 @@ def proton_32c64(torch, pc, dev, steps, warmup, hip, timer):
         gbs = bps * V / (kern * 1e-3) / 1e9
         res[name] = {"ms_per_step": wall, "kernel_ms": kern, "sites_per_s": V / (wall * 1e-3),
                      "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        if name == "fp64":
+            ref = out.cpu().numpy()
+        else:
+            res[name]["per_timeslice_rel_err_vs_fp64"] = per_timeslice_error(out.cpu().numpy(), ref)
         del ps
     res["fp32_speedup"] = res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"]
+    res["fp32_scaled_speedup"] = res["fp64"]["ms_per_step"] / res["fp32_scaled"]["ms_per_step"]
+    res["fp32_scaled_kernel_speedup"] = res["fp64"]["kernel_ms"] / res["fp32_scaled"]["kernel_ms"]
     return res
 
 

@@ -34,6 +34,8 @@ EXT_SYMBOLS = (
     "contract_baryon_twopoint_project",
     "contract_baryon_twopoint_project_sub",
     "contract_baryon_batch_workspace_bytes",
+    "contract_scaled_fp32_bytes",
+    "contract_propagator_to_scaled_fp32",
     "contract_baryon_twopoint_project_batch",
     "contract_set_kernel_timer",
     "contract_set_stream",
@@ -83,6 +85,10 @@ def lib() -> ctypes.CDLL:
     L.contract_baryon_twopoint_project.restype = I
     L.contract_baryon_twopoint_project_sub.argtypes = [P, P, P, P, I, P, P, I, I, I, P, I, P, I, P, P, S, P]
     L.contract_baryon_twopoint_project_sub.restype = I
+    L.contract_scaled_fp32_bytes.argtypes = [S]
+    L.contract_scaled_fp32_bytes.restype = S
+    L.contract_propagator_to_scaled_fp32.argtypes = [P, P, S, P]
+    L.contract_propagator_to_scaled_fp32.restype = I
     L.contract_baryon_batch_workspace_bytes.argtypes = [P, I, I]
     L.contract_baryon_batch_workspace_bytes.restype = S
     L.contract_baryon_twopoint_project_batch.argtypes = [P, I, I, P, P, I, I, I, P, I, P, I, P, S, P]

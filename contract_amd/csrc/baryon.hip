@@ -505,7 +505,7 @@ template <typename Real, int NT> constexpr bool project_persistent()
   return CONTRACT_PROJECT_PERSIST && sizeof(Real) == 8 && NT <= 2;
 }
 
-template <int NT, typename Real, int GIJ = -1>
+template <int NT, typename Real, int GIJ = -1, bool SCALED = false>
 __global__ __launch_bounds__(WAVE)
 __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : (NT == 1 ? CONTRACT_WPE32 : 1)))) void
 baryon_project_kernel(
@@ -531,13 +531,21 @@ baryon_project_kernel(
     const size_t left = end - s;
     return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
   };
+  // FP32 site-scaled storage: e_i + e_j + e_n of this lane's site in the
+  // tile at st (padded lanes repeat the last site; their R is zero)
+  auto site_exp = [&](const ProjectReq &rq, size_t st, int n) {
+    const size_t k = st + (size_t)(g < n ? g : n - 1);
+    return (int)rq.exp_i[k] + (int)rq.exp_j[k] + (int)rq.exp_n[k];
+  };
   Tile<C> buf[3];
+  int ex_buf = 0; // exponent sum of the tile in buf
   {
     const ProjectReq &pa = args.req[r];
     const int n = tile_sites(first, last);
     buf[0].load((const C *)pa.propag_i, first, n, lane);
     buf[1].load((const C *)pa.propag_j, first, n, lane);
     buf[2].load((const C *)pa.propag_n, first, n, lane);
+    if constexpr (SCALED) ex_buf = site_exp(pa, first, n);
   }
 
   for (;;) {
@@ -567,6 +575,7 @@ baryon_project_kernel(
 #pragma unroll
       for (int p = 0; p < 3; ++p) buf[p].store(lds[p], lane);
       __syncthreads();
+      int ex = ex_buf;
       const size_t next = s + SITES_PER_WAVE;
       const bool here = next < last; // else the next item's first tile (uniform)
       if (here || more) {
@@ -575,11 +584,11 @@ baryon_project_kernel(
         buf[0].load(here ? Pi : Qi, st, n, lane);
         buf[1].load(here ? Pj : Qj, st, n, lane);
         buf[2].load(here ? Pn : Qn, st, n, lane);
+        if constexpr (SCALED) ex_buf = site_exp(here ? pa : args.req[r_next], st, n);
       }
 
       C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[2] + g * SL;
-      int ex = 0;
-      if constexpr (sizeof(Real) == 4) {
+      if constexpr (sizeof(Real) == 4 && !SCALED) {
         // P_i and P_j only: D of the rescaled blocks is O(1), so the tails'
         // products with P_n are as large as P_n's own entries and cannot
         // underflow unless a stored entry already is subnormal
@@ -613,37 +622,32 @@ baryon_project_kernel(
   }
 }
 
-// Resident blocks per CU of a projection kernel, queried once per device
-// with the largest phase table.
-template <typename K> static unsigned resident_waves(K kernel)
-{
-  static std::atomic<int> cache[64];
-  int dev = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-  if ((per_cu = cache[dev].load(std::memory_order_relaxed)) <= 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, WAVE, sizeof(double2) * MAX_PHASE_TAB) !=
-          hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    cache[dev].store(per_cu, std::memory_order_relaxed);
-  }
-  return (unsigned)per_cu * (unsigned)device_cu_count();
-}
-
-template <int NT, typename Real, int GIJ>
+template <int NT, typename Real, int GIJ, bool SCALED>
 static void launch_project_nt(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t tab_bytes,
                               hipStream_t stream)
 {
-  auto kernel = baryon_project_kernel<NT, Real, GIJ>;
+  auto kernel = baryon_project_kernel<NT, Real, GIJ, SCALED>;
   unsigned grid = pa.total_items;
-  if (project_persistent<Real, NT>()) { // as many waves as the chip holds at once
-    const unsigned w = resident_waves(kernel);
+  if (project_persistent<Real, NT>()) {
+    // as many waves as the chip holds at once: resident blocks per CU (queried
+    // once per device, with the largest phase table) x CUs
+    static std::atomic<int> cache[64];
+    int dev = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if ((per_cu = cache[dev].load(std::memory_order_relaxed)) <= 0) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, WAVE, sizeof(double2) * MAX_PHASE_TAB) !=
+            hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+      cache[dev].store(per_cu, std::memory_order_relaxed);
+    }
+    const unsigned w = (unsigned)per_cu * (unsigned)device_cu_count();
     grid = grid < w ? grid : w;
   }
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(WAVE), tab_bytes, stream, a, pa);
 }
 
-template <typename Real, int GIJ>
+template <typename Real, int GIJ, bool SCALED>
 static void launch_project_g(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
 {
   const int nt = (pa.np + 15) / 16;
@@ -654,20 +658,20 @@ static void launch_project_g(const BaryonProjectArgs &a, const ProjectArgs &pa, 
   }
   const size_t tab_bytes = sizeof(double2) * tab;
   switch (nt) {
-  case 1: launch_project_nt<1, Real, GIJ>(a, pa, tab_bytes, stream); break;
-  case 2: launch_project_nt<2, Real, GIJ>(a, pa, tab_bytes, stream); break;
-  case 3: launch_project_nt<3, Real, GIJ>(a, pa, tab_bytes, stream); break;
-  default: launch_project_nt<4, Real, GIJ>(a, pa, tab_bytes, stream); break;
+  case 1: launch_project_nt<1, Real, GIJ, SCALED>(a, pa, tab_bytes, stream); break;
+  case 2: launch_project_nt<2, Real, GIJ, SCALED>(a, pa, tab_bytes, stream); break;
+  case 3: launch_project_nt<3, Real, GIJ, SCALED>(a, pa, tab_bytes, stream); break;
+  default: launch_project_nt<4, Real, GIJ, SCALED>(a, pa, tab_bytes, stream); break;
   }
 }
 
-template <typename Real>
+template <typename Real, bool SCALED = false>
 static void launch_project_t(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
 {
   if (a.gamma_ij == 5)
-    launch_project_g<Real, 5>(a, pa, stream);
+    launch_project_g<Real, 5, SCALED>(a, pa, stream);
   else
-    launch_project_g<Real, -1>(a, pa, stream);
+    launch_project_g<Real, -1, SCALED>(a, pa, stream);
 }
 
 void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
@@ -678,6 +682,11 @@ void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, hi
 void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
 {
   launch_project_t<float>(a, pa, stream);
+}
+
+void launch_baryon_project_fp32_scaled(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
+{
+  launch_project_t<float, true>(a, pa, stream);
 }
 
 } // namespace contract_amd

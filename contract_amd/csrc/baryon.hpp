@@ -41,6 +41,7 @@ constexpr int MAX_BATCH = 16; // requests (propagator triples) per fused launch
 // timeslice, or one parity of it), so every item feeds exactly one timeslice.
 struct ProjectReq {
   const void *propag_i, *propag_j, *propag_n;
+  const signed char *exp_i, *exp_j, *exp_n; // site exponents (FP32 site-scaled storage only)
   double2 *partial;   // [items][np slots][2][16]: C and S of each momentum slot
   size_t slice_sites; // sites per (parity, t) slice
   int dims[4];        // Lx, Ly, Lz, Lt
@@ -75,6 +76,20 @@ void launch_baryon_project(const BaryonProjectArgs &a, const ProjectArgs &pa, hi
 // Same with complex64 propagators: FP32 per-site contraction on blocks rescaled
 // by exact powers of two, FP64 projection and sums.
 void launch_baryon_project_fp32(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream);
+// Same on FP32 site-scaled propagators: no per-site rescaling in the kernel,
+// 2^(e_i + e_j + e_n) applied to each site's result in FP64.
+void launch_baryon_project_fp32_scaled(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream);
+
+// ---- FP32 site-scaled storage -------------------------------------------
+constexpr int SCALED_EXP_MAX = 127;      // site exponents are int8 in [-127, 127]
+constexpr size_t SCALED_SITE_BYTES = 1152; // complex64 [4][4][3][3]; the exponents follow all V sites
+struct ScaleArgs {
+  const void *src; // complex128 [V][4][4][3][3]
+  void *dst;       // complex64 [V][4][4][3][3]
+  void *exps;      // int8 [V]
+  size_t volume;
+};
+void launch_scale_fp32(const ScaleArgs &a, hipStream_t stream);
 
 // partial[items][np][nq] -> open[t][np][nq]: per-timeslice sum over the
 // timeslice's work items (both parities), fixed order, Neumaier compensated.

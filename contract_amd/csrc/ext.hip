@@ -345,8 +345,8 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
     }
   }
   if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
-  if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32) {
-    set_last_error("precision %d is not CONTRACT_FP64 or CONTRACT_FP32", precision);
+  if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32 && precision != CONTRACT_FP32_SCALED) {
+    set_last_error("precision %d is not CONTRACT_FP64, CONTRACT_FP32 or CONTRACT_FP32_SCALED", precision);
     return CONTRACT_ERR_ARG;
   }
   if (nterm < 1 || nterm > MAX_TERMS || !contract_types || !weights) {
@@ -413,6 +413,12 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
     R.propag_i = reqs[r].propag_i;
     R.propag_j = reqs[r].propag_j;
     R.propag_n = reqs[r].propag_n;
+    if (precision == CONTRACT_FP32_SCALED) { // the site exponents follow the complex64 entries
+      const size_t V = (size_t)geo[r].dims[0] * geo[r].dims[1] * geo[r].dims[2] * geo[r].dims[3];
+      R.exp_i = static_cast<const signed char *>(R.propag_i) + V * SCALED_SITE_BYTES;
+      R.exp_j = static_cast<const signed char *>(R.propag_j) + V * SCALED_SITE_BYTES;
+      R.exp_n = static_cast<const signed char *>(R.propag_n) + V * SCALED_SITE_BYTES;
+    }
     R.partial = reinterpret_cast<double2 *>(wsb + offset[r]);
     R.slice_sites = plan[r].slice_sites;
     for (int mu = 0; mu < 4; ++mu) R.dims[mu] = geo[r].dims[mu];
@@ -442,8 +448,10 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
     pa.total_items = (unsigned)items;
     if (precision == CONTRACT_FP64)
       launch_baryon_project(a, pa, stream);
-    else
+    else if (precision == CONTRACT_FP32)
       launch_baryon_project_fp32(a, pa, stream);
+    else
+      launch_baryon_project_fp32_scaled(a, pa, stream);
     CONTRACT_EXT_HIP(hipGetLastError());
   }
   CONTRACT_EXT_HIP(timer_end(stream));
@@ -467,6 +475,24 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
   }
   CONTRACT_EXT_HIP(hipGetLastError());
   if (own) CONTRACT_EXT_HIP(hipFreeAsync(ws, stream));
+  return CONTRACT_OK;
+}
+
+size_t contract_scaled_fp32_bytes(size_t volume) { return volume * (SCALED_SITE_BYTES + 1); }
+
+int contract_propagator_to_scaled_fp32(void *dst, const void *src, size_t volume, void *stream)
+{
+  if (!dst || !src || volume == 0) {
+    set_last_error("NULL propagator or zero volume");
+    return CONTRACT_ERR_ARG;
+  }
+  ScaleArgs a;
+  a.src = src;
+  a.dst = dst;
+  a.exps = static_cast<char *>(dst) + volume * SCALED_SITE_BYTES;
+  a.volume = volume;
+  launch_scale_fp32(a, static_cast<hipStream_t>(stream));
+  CONTRACT_EXT_HIP(hipGetLastError());
   return CONTRACT_OK;
 }
 

@@ -60,6 +60,7 @@ class BaryonSequentialType(enum.IntEnum):
 
 CONTRACT_FP64 = 0
 CONTRACT_FP32 = 1
+CONTRACT_FP32_SCALED = 2
 LEXICOGRAPHIC = 0
 EVEN_ODD = 1
 
@@ -139,7 +140,62 @@ def gamma_matrix(g: int) -> np.ndarray:
     return out
 
 
+class ScaledPropagator:
+    """A propagator in the FP32 site-scaled form (include/contract_ext.h):
+    `data` is a uint8 device tensor of V * 1153 bytes, complex64 P(x) 2^-e(x)
+    followed by the int8 site exponents e(x)."""
+
+    def __init__(self, data, volume: int):
+        self.data, self.volume = data, int(volume)
+        self.device = data.device
+
+    def data_ptr(self) -> int:
+        return self.data.data_ptr()
+
+    def entries(self):
+        """The stored complex64 [V, 4, 4, 3, 3] (scaled) entries, a view."""
+        return self.data[: self.volume * 1152].view(dtype=__import__("torch").complex64).view(-1, 4, 4, 3, 3)
+
+    def exponents(self):
+        """The int8 [V] site exponents, a view."""
+        return self.data[self.volume * 1152:].view(dtype=__import__("torch").int8)
+
+
+def scaled_fp32_bytes(volume: int) -> int:
+    return int(lib().contract_scaled_fp32_bytes(int(volume)))
+
+
+def toScaledFP32(propag, out=None, stream=None) -> ScaledPropagator:
+    """complex128 [V, 4, 4, 3, 3] device propagator -> ScaledPropagator."""
+    import torch
+    if propag.dtype != torch.complex128:
+        raise TypeError("toScaledFP32 converts complex128 propagators")
+    V = _volume(propag)
+    if out is None:
+        out = torch.empty(scaled_fp32_bytes(V), dtype=torch.uint8, device=propag.device)
+    rc = lib().contract_propagator_to_scaled_fp32(_ptr(out), _ptr(propag), V, 0 if stream is None else int(stream))
+    if rc != 0:
+        raise RuntimeError(f"contract_propagator_to_scaled_fp32 failed ({rc}): {last_error()}")
+    return ScaledPropagator(out, V)
+
+
+def _precision(props) -> int:
+    """Precision flag of a set of propagators that must share one storage form."""
+    import torch
+    kinds = {"scaled" if isinstance(p, ScaledPropagator) else str(p.dtype) for p in props}
+    if len(kinds) != 1:
+        raise TypeError(f"propagators of one call must share a storage form, got {sorted(kinds)}")
+    kind = kinds.pop()
+    if kind == "scaled":
+        return CONTRACT_FP32_SCALED
+    for p in props:
+        _volume(p)
+    return CONTRACT_FP32 if kind == str(torch.complex64) else CONTRACT_FP64
+
+
 def _volume(p) -> int:
+    if isinstance(p, ScaledPropagator):
+        return p.volume
     if p.dtype.__str__() not in ("torch.complex128", "torch.complex64"):
         raise TypeError(f"propagators must be complex128/complex64 tensors, got {p.dtype}")
     if not p.is_contiguous():
@@ -249,16 +305,13 @@ def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, ga
     for every insertion (gamma index or 4x4 projector), projected to
     ``out[len(insertions), len(momenta), Lt]`` (complex128, on the device).
 
-    Propagators may be complex128 (FP64 path) or complex64 (FP32 contraction,
-    FP64 accumulation).  global_dims / origin place the local lattice inside a
+    Propagators may be complex128 (FP64 path), complex64 (FP32 contraction,
+    FP64 accumulation) or ScaledPropagator (FP32 site-scaled form, see
+    toScaledFP32).  global_dims / origin place the local lattice inside a
     larger one (one rank of an MPI grid): phases use global coordinates and
     momenta are in units of 2pi / global extent."""
     import torch
-    precision = CONTRACT_FP32 if propag_i.dtype == torch.complex64 else CONTRACT_FP64
-    for p in (propag_i, propag_j, propag_n):
-        if p.dtype != propag_i.dtype:
-            raise TypeError("all three propagators must have the same dtype")
-        _volume(p)
+    precision = _precision((propag_i, propag_j, propag_n))
     W = _projectors(insertions)
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
     types = np.ascontiguousarray([int(t) for t, _ in terms], dtype=np.int32)
@@ -331,13 +384,7 @@ def baryonTwoPointProjectBatch(requests, terms, gamma_ij: int, gamma_kl: int, in
     import torch
     if not requests:
         return []
-    dtype = requests[0][0].dtype
-    precision = CONTRACT_FP32 if dtype == torch.complex64 else CONTRACT_FP64
-    for req in requests:
-        for p in req[:3]:
-            if p.dtype != dtype:
-                raise TypeError("all propagators of a batch must have the same dtype")
-            _volume(p)
+    precision = _precision([p for req in requests for p in req[:3]])
     W = _projectors(insertions)
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
     types = np.ascontiguousarray([int(t) for t, _ in terms], dtype=np.int32)

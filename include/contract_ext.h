@@ -27,6 +27,9 @@
  *               CONTRACT_FP32: propagators are complex64; the per-site
  *               contraction runs in FP32 on power-of-two rescaled blocks and
  *               the timeslice/momentum sums stay FP64 (compensated).
+ *               CONTRACT_FP32_SCALED: propagators in the FP32 site-scaled form
+ *               of contract_propagator_to_scaled_fp32(); FP32 contraction with
+ *               no per-site rescaling in the kernel, FP64 sums.
  *   workspace   device scratch of contract_project_workspace_bytes() bytes,
  *               or NULL to let the library allocate it stream-ordered.
  *   stream      a hipStream_t (NULL = null stream).
@@ -46,11 +49,26 @@
 extern "C" {
 #endif
 
-enum { CONTRACT_FP64 = 0, CONTRACT_FP32 = 1 };
+enum { CONTRACT_FP64 = 0, CONTRACT_FP32 = 1, CONTRACT_FP32_SCALED = 2 };
 enum { CONTRACT_LEXICOGRAPHIC = 0, CONTRACT_EVEN_ODD = 1 };
 
 /* Message of the calling thread's most recent error ("" if none). */
 const char *contract_last_error(void);
+
+/* FP32 site-scaled propagator storage (CONTRACT_FP32_SCALED), V sites in the
+ * caller's site order: complex64 [V][4][4][3][3] holding P(x) * 2^-e(x),
+ * followed by int8 e(x)[V], where e(x) = floor(log2 of the largest |Re| or
+ * |Im| of site x) clamped to [-127, 127] (0 for an all-zero site), so every
+ * stored site's largest component lies in [1, 2).  A site whose components
+ * all lie within 2^-127 .. 2^128 is stored to FP32 rounding; the contraction
+ * applies 2^(e_i + e_j + e_n) to each site's result in FP64.  The form holds
+ * propagators decaying over many decades without FP32 underflow in the
+ * products of three of them. */
+size_t contract_scaled_fp32_bytes(size_t volume); /* V * 1153 */
+
+/* complex128 [V][4][4][3][3] (device) -> FP32 site-scaled form (device, of
+ * contract_scaled_fp32_bytes(V) bytes), stream-ordered. */
+int contract_propagator_to_scaled_fp32(void *dst, const void *src, size_t volume, void *stream);
 
 /* Dense DeGrand-Rossi G(g), g = 0..15, written as [4][4] row-major. */
 int contract_gamma_matrix(int g, double _Complex *out);

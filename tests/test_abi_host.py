@@ -159,6 +159,14 @@ def test_baryon_batch_validation_and_workspace():
     assert L.contract_baryon_batch_workspace_bytes(ctypes.cast(a, vp), 2, 0) == 0
 
 
+def test_scaled_fp32_form_size_and_validation():
+    import contract_amd
+    L = contract_amd.lib()
+    assert L.contract_scaled_fp32_bytes(1000) == 1153 * 1000
+    assert L.contract_propagator_to_scaled_fp32(None, None, 10, None) != 0
+    assert "NULL" in contract_amd.last_error()
+
+
 def test_meson_grid_rejects_bad_arguments_with_codes():
     import ctypes
     import numpy as np
