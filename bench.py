@@ -516,6 +516,24 @@ def main():
 
     wall32, kern32 = time_steps(torch, step32, args.steps, args.warmup, hip, timer)
     err32 = per_timeslice_error(out32.cpu().numpy(), ref64)
+    del u_f, d_f, u2_f
+
+    # FP32 site-scaled storage form of the same step (conversion timed apart)
+    conv = [None, None, None]
+
+    def convert():
+        for k, p in enumerate((u, d, u2)):
+            conv[k] = pc.toScaledFP32(p, out=None if conv[k] is None else conv[k].data)
+
+    wall_conv, _ = time_steps(torch, convert, 5, 1)
+    out32s = torch.empty_like(out64)
+
+    def step32s():
+        pc.baryonTwoPointProject(*conv, TERMS, GAMMA_CG5, GAMMA_CG5, gammas, MOM27, DIMS, out=out32s, workspace=ws)
+
+    wall32s, kern32s = time_steps(torch, step32s, args.steps, args.warmup, hip, timer)
+    err32s = per_timeslice_error(out32s.cpu().numpy(), ref64)
+    del conv
 
     # drop-in per-site kernel (one insertion, one term), reference-style output
     correl = torch.empty(V, dtype=torch.complex128, device=dev)
@@ -526,7 +544,6 @@ def main():
     wall_abi, _ = time_steps(torch, step_abi, args.steps, args.warmup)
     abi_all = abi_entry_points(torch, pc, u, d, u2, V, max(5, args.steps // 5), args.warmup)
 
-    del u_f, d_f, u2_f
     precision = None
     if not args.no_precision:
         del ws
@@ -593,6 +610,18 @@ def main():
             "speedup_vs_fp64": wall64 / wall32,
             "kernel_speedup_vs_fp64": kern64 / kern32,
             "per_timeslice_rel_err_vs_fp64": err32,
+        },
+        "fp32_scaled": {
+            "note": "FP32 site-scaled storage (contract_propagator_to_scaled_fp32): no per-site rescaling in "
+                    "the kernel; conversion of the three propagators timed apart",
+            "sites_per_s": V / (wall32s * 1e-3),
+            "ms_per_step": wall32s,
+            "kernel_ms": kern32s,
+            "achieved_GBs": (BYTES_PER_SITE_FP32 + 3) * V / (kern32s * 1e-3) / 1e9,
+            "speedup_vs_fp64": wall64 / wall32s,
+            "kernel_speedup_vs_fp64": kern64 / kern32s,
+            "per_timeslice_rel_err_vs_fp64": err32s,
+            "convert_ms_three_propagators": wall_conv,
         },
         "abi_baryon_two_point": {
             "ms_per_call": wall_abi,

@@ -104,3 +104,20 @@ def test_mixed_storage_forms_are_rejected(dev):
     p = torch.from_numpy(oracle.synthetic_propagator(dims, 1, 0.1)).to(cuda)
     with pytest.raises(TypeError):
         pc.baryonTwoPointProject(pc.toScaledFP32(p), p, p, PROTON, 5, 5, [0], [(0, 0, 0)], dims)
+
+
+@pytest.mark.parametrize("scaled", [False, True])
+def test_nucleon_kernel_equals_runtime_terms(dev, scaled):
+    """(IK_JL_NM, IM_JL_NK) with Cg5 takes the compile-time nucleon kernel in
+    FP32; the same terms in the other order take the runtime-dispatch kernel.
+    Both sum two FP32 tails in FP64, which is exact, so they agree to 1e-12."""
+    torch, pc, cuda = dev
+    dims = (4, 4, 2, 6)
+    ps = [oracle.synthetic_propagator(dims, s, 0.3, t0=2) for s in (61, 62, 63)]
+    d = [torch.from_numpy(p).to(cuda) for p in ps]
+    d = [pc.toScaledFP32(p) for p in d] if scaled else [p.to(torch.complex64) for p in d]
+    a = pc.baryonTwoPointProject(*d, [(0, 1.0), (5, 0.5)], 5, 5, list(range(16)), MOM27, dims).cpu().numpy()
+    b = pc.baryonTwoPointProject(*d, [(5, 0.5), (0, 1.0)], 5, 5, list(range(16)), MOM27, dims).cpu().numpy()
+    assert per_timeslice_error(a, b) < 1e-12
+    want = oracle.baryon_twopoint_project(*ps, [(0, 1.0), (5, 0.5)], 5, 5, list(range(16)), dims, MOM27)
+    assert per_timeslice_error(a, want) < 1e-6
