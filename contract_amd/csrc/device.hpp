@@ -230,7 +230,8 @@ template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
   static constexpr int PER_LANE = (CHUNKS + WAVE - 1) / WAVE;         // 4 sites: 9 / 5; 2 sites: 5 / 3
   uint4 v[PER_LANE];
 
-  // Chunks past nsites (ragged tile) and lanes past the tile read as zero.
+  // Chunks past nsites (ragged tile) and lanes past the tile read as zero;
+  // nsites = 0 reads an all-zero tile without touching memory.
   __device__ __forceinline__ void load(const C *__restrict__ src, size_t first_site, int nsites, int lane)
   {
     const __amdgpu_buffer_rsrc_t rsrc = tile_rsrc(src + first_site * SITE_ELEMS, (unsigned)(nsites * SITE16 * 16));

@@ -6,16 +6,18 @@ LRU pool with a byte cap, admits a waiting request when a launch slot is free
 and the pool can hold everything the request needs (its missing inputs and
 its outputs) without evicting anything in use, and batches continuously:
 every scheduler tick, the meson requests admitted together go through one
-batched persistent launch (contract_meson_grid_project), baryon and diquark
-requests get a launch each, and launches are spread over several streams so
-that work admitted while earlier work is still on the GPU overlaps with it.
+batched persistent launch (contract_meson_grid_project), the baryon requests
+through one batched fused launch (contract_baryon_twopoint_project_batch),
+diquark requests get a launch each, and launches are spread over several
+streams so that work admitted while earlier work is still on the GPU
+overlaps with it.
 A request completes when the event recorded after its last kernel has
 passed; its latency is completion minus arrival.
 
 The results do not depend on what else is in flight: each request's kernels
 see the same inputs and the same work decomposition alone and in the mix (the
-batched meson launch computes every request from its own work items and
-reduces them in a fixed order), which the tests check.
+batched launches compute every request from its own work items and reduce
+them in a fixed order), which the tests check.
 
 Nothing here runs on the CPU in place of the GPU: every contraction is a
 libcontract.so launch.
@@ -245,7 +247,7 @@ class Engine:
         self.pool = PropagatorPool(cap_bytes, self._materialise)
         self._rr = 0
         self._spare: Dict[int, Entry] = {}
-        self.launches = {"meson_batches": 0, "meson_requests": 0, "baryon": 0, "diquark": 0}
+        self.launches = {"meson_batches": 0, "meson_requests": 0, "baryon_batches": 0, "baryon": 0, "diquark": 0}
 
     # -- pool ----------------------------------------------------------------
     def _materialise(self, key, recycled=None):
@@ -321,16 +323,36 @@ class Engine:
         self.launches["meson_requests"] += len(reqs)
         return keep
 
-    def _launch_baryon(self, r: Request, stream) -> list:
+    def _launch_baryons(self, reqs: List[Request], stream) -> list:
+        """All baryon requests of a tick in batched fused launches
+        (contract_baryon_twopoint_project_batch, up to 16 per launch) over the
+        union of their insertions and momenta; every request then gathers
+        its own (insertion, momentum) entries.  Momenta of {-1, 0, 1}^3 pair
+        into at most 14 +-k slots, one MFMA tile, so the union costs the
+        kernel nothing."""
         torch = self.torch
+        keep = []
+        gammas = sorted({int(g) for r in reqs for g in r.gammas})
+        moms = sorted({tuple(m) for r in reqs for m in r.momenta})
+        by_dtype: Dict[str, List[Request]] = {}
+        for r in reqs:
+            by_dtype.setdefault(r.props[0].dtype, []).append(r)
         with torch.cuda.stream(stream):
-            u, d, u2 = self._inputs(r, stream)
-            ws = torch.empty(pc.project_workspace_bytes(r.dims, len(r.momenta)), dtype=torch.uint8,
-                             device=self.device)
-            r.result = pc.baryonTwoPointProject(u, d, u2, PROTON, CG5, CG5, list(r.gammas), list(r.momenta), r.dims,
-                                                workspace=ws, stream=stream.cuda_stream)
-        self.launches["baryon"] += 1
-        return [ws]
+            for dt, group in by_dtype.items():
+                batch = [(*self._inputs(r, stream), r.dims) for r in group]
+                ws = torch.empty(pc.baryon_batch_workspace_bytes(batch, len(moms)), dtype=torch.uint8,
+                                 device=self.device)
+                outs = pc.baryonTwoPointProjectBatch(batch, PROTON, CG5, CG5, gammas, moms, workspace=ws,
+                                                     stream=stream.cuda_stream)
+                keep += [ws] + outs
+                for r, o in zip(group, outs):
+                    gi = self._index([gammas.index(int(g)) for g in r.gammas])
+                    mi = self._index([moms.index(tuple(m)) for m in r.momenta])
+                    r.result = o.index_select(0, gi).index_select(1, mi)
+                    keep += [gi, mi]
+        self.launches["baryon_batches"] += 1
+        self.launches["baryon"] += len(reqs)
+        return keep
 
     def _launch_diquark(self, r: Request, stream) -> list:
         torch = self.torch
@@ -399,10 +421,10 @@ class Engine:
             flights = alive
             admitted: List[Request] = []
             slots = self.max_inflight - len(flights)
-            nmeson = 0
+            nbatch = {"meson": 0, "baryon": 0}
             while queue and slots > 0:
                 r = queue[0]
-                if (r.kind == "meson" and nmeson == self.max_batch) or not self._admissible(r):
+                if (r.kind in nbatch and nbatch[r.kind] == self.max_batch) or not self._admissible(r):
                     break
                 queue.popleft()
                 for k in r.props:
@@ -413,26 +435,28 @@ class Engine:
                         self._spare[r.rid] = spare
                 r.admitted = time.perf_counter() - t_start
                 admitted.append(r)
-                if r.kind != "meson":
+                if r.kind not in nbatch:
                     slots -= 1
-                else:
-                    nmeson += 1
-                    slots -= nmeson == 1  # one batched launch for all mesons of this tick
+                else:  # one batched launch for all mesons (baryons) of this tick
+                    nbatch[r.kind] += 1
+                    slots -= nbatch[r.kind] == 1
             if admitted:
                 progressed = True
-                mesons = [r for r in admitted if r.kind == "meson"]
-                if mesons:
-                    s = self._next_stream()
-                    keep = self._launch_mesons(mesons, s)
-                    ev = torch.cuda.Event()
-                    ev.record(s)
-                    flights.append(Flight(ev, mesons, keep))
-                    batch_sizes.append(len(mesons))
+                for kind, launch in (("meson", self._launch_mesons), ("baryon", self._launch_baryons)):
+                    group = [r for r in admitted if r.kind == kind]
+                    if group:
+                        s = self._next_stream()
+                        keep = launch(group, s)
+                        ev = torch.cuda.Event()
+                        ev.record(s)
+                        flights.append(Flight(ev, group, keep))
+                        if kind == "meson":
+                            batch_sizes.append(len(group))
                 for r in admitted:
-                    if r.kind == "meson":
+                    if r.kind != "diquark":
                         continue
                     s = self._next_stream()
-                    keep = self._launch_baryon(r, s) if r.kind == "baryon" else self._launch_diquark(r, s)
+                    keep = self._launch_diquark(r, s)
                     ev = torch.cuda.Event()
                     ev.record(s)
                     flights.append(Flight(ev, [r], keep))

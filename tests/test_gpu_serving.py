@@ -68,6 +68,7 @@ def test_mixed_stream_equals_alone_and_oracle(srv):
     eng = serve(torch, serving, reqs, cap_bytes=1 << 34)
     assert all(r.done is not None for r in reqs)
     assert eng.launches["meson_batches"] < eng.launches["meson_requests"]  # mesons were batched
+    assert eng.launches["baryon_batches"] < eng.launches["baryon"]  # and baryons
     mixed = {r.rid: snapshot(torch, eng, r) for r in reqs}
     del eng
     for r in reqs:
