@@ -234,7 +234,7 @@ class Engine:
 
     def __init__(self, cap_bytes: int = 200 << 30, streams: int = 4, max_inflight: int = 8, device="cuda:0",
                  grid_tuning: Optional[Dict[str, Tuple[int, int, int]]] = None, poll_s: float = 5e-5,
-                 max_batch: int = 32):
+                 max_batch: int = 32, batch_baryons: bool = True):
         import torch
         self.torch = torch
         self.device = torch.device(device)
@@ -242,6 +242,7 @@ class Engine:
         self.load_stream = torch.cuda.Stream(device=self.device)
         self.max_inflight = max_inflight
         self.max_batch = max_batch
+        self.batch_baryons = batch_baryons
         self.grid_tuning = dict(GRID_TUNING, **(grid_tuning or {}))
         self.poll_s = poll_s
         self.pool = PropagatorPool(cap_bytes, self._materialise)
@@ -435,18 +436,29 @@ class Engine:
                         self._spare[r.rid] = spare
                 r.admitted = time.perf_counter() - t_start
                 admitted.append(r)
-                if r.kind not in nbatch:
+                if r.kind == "diquark" or (r.kind == "baryon" and not self.batch_baryons):
                     slots -= 1
-                else:  # one batched launch for all mesons (baryons) of this tick
-                    nbatch[r.kind] += 1
-                    slots -= nbatch[r.kind] == 1
+                elif r.kind == "meson":  # one batched launch for all mesons of this tick
+                    nbatch["meson"] += 1
+                    slots -= nbatch["meson"] == 1
+                else:  # one batched launch per lattice for the baryons of this tick
+                    nbatch["baryon"] += 1
+                    slots -= not any(a.kind == "baryon" and a.dims == r.dims for a in admitted[:-1])
             if admitted:
                 progressed = True
-                for kind, launch in (("meson", self._launch_mesons), ("baryon", self._launch_baryons)):
-                    group = [r for r in admitted if r.kind == kind]
+                # mesons: one flight for the tick; baryons: one flight per
+                # lattice, so a batch on a small lattice completes (and
+                # releases its pinned propagators) without waiting for a large one
+                groups = [("meson", [r for r in admitted if r.kind == "meson"])]
+                if self.batch_baryons:
+                    for dims in sorted({r.dims for r in admitted if r.kind == "baryon"}):
+                        groups.append(("baryon", [r for r in admitted if r.kind == "baryon" and r.dims == dims]))
+                else:
+                    groups += [("baryon", [r]) for r in admitted if r.kind == "baryon"]
+                for kind, group in groups:
                     if group:
                         s = self._next_stream()
-                        keep = launch(group, s)
+                        keep = (self._launch_mesons if kind == "meson" else self._launch_baryons)(group, s)
                         ev = torch.cuda.Event()
                         ev.record(s)
                         flights.append(Flight(ev, group, keep))
@@ -545,10 +557,12 @@ def make_requests(rate: float, duration: float, seed: int, lattices=LATTICES, ca
 
 
 def sweep(rates: Sequence[float], duration: float, seed: int = 7, cap_gb: float = 200.0, streams: int = 4,
-          max_inflight: int = 8, lattices=LATTICES, catalogue: int = 4, warm: bool = True) -> Dict[str, object]:
+          max_inflight: int = 8, lattices=LATTICES, catalogue: int = 4, warm: bool = True,
+          batch_baryons: bool = True) -> Dict[str, object]:
     """Serve one Poisson stream per rate on a shared engine (the pool stays warm
     between rates) and report throughput and latency per rate."""
-    eng = Engine(cap_bytes=int(cap_gb * 1e9), streams=streams, max_inflight=max_inflight)
+    eng = Engine(cap_bytes=int(cap_gb * 1e9), streams=streams, max_inflight=max_inflight,
+                 batch_baryons=batch_baryons)
     if warm:  # load the catalogue, largest lattices last (most recently used)
         for dims in lattices:
             for i in range(catalogue):
