@@ -160,13 +160,28 @@ def cpu_baseline(torch, props, gpu_fn, target_s=10.0):
     sites = V3 * nt
     dev = [torch.from_numpy(np.ascontiguousarray(s)).to(props[0].device) for s in sample]
     got = gpu_fn(dev, dims).cpu().numpy()
+    cores = oracle.num_threads()
+    # one core on one timeslice (a few seconds): the per-core rate
+    oracle.set_num_threads(1)
+    dt1, _, _, _ = run(1)
+    oracle.set_num_threads(0)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": sites / dt,
         "unit": "sites/s",
-        "cores": oracle.num_threads(),
+        "cores": cores,
         "kind": "port",
         "sample": f"first {nt} timeslices ({sites} sites) of the same 24^3x64 proton workload (16 insertions, "
                   f"27 momenta), FP64 oracle, {dt:.1f} s wall",
+        "single_core_value": V3 / dt1,
+        "single_core_sample": f"first timeslice ({V3} sites), 1 thread, {dt1:.1f} s wall",
+        "cpu_model": model,
+        "nproc": os.cpu_count(),
         "gpu_vs_oracle_rel_err": per_timeslice_error(got, want),
     }
 

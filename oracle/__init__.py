@@ -57,6 +57,8 @@ def lib():
         L.oracle_project.argtypes = [P, P, I, P, I, P, I, P, P]
         L.oracle_baryon_twopoint_project.argtypes = [P, P, P, P, P, P, I, I, I, P, I, P, I, P, I, P, P]
         L.oracle_num_threads.restype = I
+        L.oracle_set_num_threads.argtypes = [I]
+        L.oracle_set_num_threads.restype = None
         _lib = L
     return _lib
 
@@ -85,6 +87,11 @@ def _volume(p):
 
 def num_threads() -> int:
     return lib().oracle_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    """Threads of the following oracle calls (n <= 0: the OpenMP default)."""
+    lib().oracle_set_num_threads(int(n))
 
 
 def gamma(n: int) -> np.ndarray:

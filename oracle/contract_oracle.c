@@ -526,3 +526,15 @@ int oracle_num_threads(void)
   return 1;
 #endif
 }
+
+/* Threads of the following calls (n <= 0: the OpenMP default). */
+void oracle_set_num_threads(int n)
+{
+#ifdef _OPENMP
+  static int dflt = 0;
+  if (!dflt) dflt = omp_get_max_threads();
+  omp_set_num_threads(n > 0 ? n : dflt);
+#else
+  (void)n;
+#endif
+}
