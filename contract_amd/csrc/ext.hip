@@ -78,6 +78,10 @@ std::vector<MomSlot> pair_momenta(const int *mom, int nmom)
 }
 
 constexpr size_t MIN_ITEM_SITES = 128;
+#ifndef CONTRACT_ITEMS_PER_CU
+#define CONTRACT_ITEMS_PER_CU 16
+#endif
+constexpr size_t ITEMS_PER_CU = CONTRACT_ITEMS_PER_CU;
 
 ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item)
 {
@@ -90,7 +94,7 @@ ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_i
     chunk = (size_t)sites_per_item;
   } else {
     // enough work items to fill the chip several times over
-    const size_t target_items = (size_t)device_cu_count() * 16;
+    const size_t target_items = (size_t)device_cu_count() * ITEMS_PER_CU;
     size_t cps = (target_items + p.slices - 1) / p.slices;
     if (cps < 1) cps = 1;
     chunk = (p.slice_sites + cps - 1) / cps;
