@@ -288,12 +288,25 @@ def precision_study(torch, pc, dev, steps, warmup, hip, timer):
                                  workspace=ws)
 
     wall64, kern64 = time_steps(torch, step64, steps, warmup, hip, timer)
+    # the FP32 site-scaled form of the same stored values (exact: the
+    # scaling is by powers of two)
+    scaled = [pc.toScaledFP32(p) for p in f64]
+    out32s = torch.empty_like(out32)
+
+    def step32s():
+        pc.baryonTwoPointProject(*scaled, terms, GAMMA_CG5, GAMMA_CG5, list(range(16)), mom, dims, out=out32s,
+                                 workspace=ws)
+
+    wall32s, kern32s = time_steps(torch, step32s, steps, warmup, hip, timer)
+    got_s = out32s.cpu().numpy()
+    del scaled
     host = [p.cpu().numpy() for p in f64]
     del f64
     t0 = time.perf_counter()
     want = oracle.baryon_twopoint_project(*host, terms, GAMMA_CG5, GAMMA_CG5, list(range(16)), dims, mom)
     t_oracle = time.perf_counter() - t0
     per_t = np.abs(got - want).max(axis=(0, 1)) / np.abs(want).max(axis=(0, 1))
+    per_t_s = np.abs(got_s - want).max(axis=(0, 1)) / np.abs(want).max(axis=(0, 1))
     return {
         "workload": "sequential-source precision study, 32^3x64, exp(-0.8|t|), FP32 storage, Cg5, 16 insertions, p=0",
         "max_per_timeslice_rel_err": float(per_t.max()),
@@ -308,6 +321,9 @@ def precision_study(torch, pc, dev, steps, warmup, hip, timer):
         "fp32_speedup": wall64 / wall32,
         "fp32_achieved_GBs": BYTES_PER_SITE_FP32 * V / (kern32 * 1e-3) / 1e9,
         "fp64_achieved_GBs": BYTES_PER_SITE_FP64 * V / (kern64 * 1e-3) / 1e9,
+        "fp32_scaled": {"max_per_timeslice_rel_err": float(per_t_s.max()),
+                        "median_per_timeslice_rel_err": float(np.median(per_t_s)),
+                        "kernel_ms": kern32s, "sites_per_s": V / (wall32s * 1e-3), "speedup": wall64 / wall32s},
         "oracle_s": t_oracle,
     }
 
@@ -613,7 +629,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "baryon_project_kernel<1,double,5>",
+            "kernel": "baryon_project_kernel<1, double, 5, false, 0>",
             "kernel_ms": kern64,
             "bytes_per_site": BYTES_PER_SITE_FP64,
         },
