@@ -141,8 +141,8 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
   if ((rc = check_tuning(g)) != CONTRACT_OK) return rc;
   if ((rc = check_requests(reqs, nreq)) != CONTRACT_OK) return rc;
   if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
-  if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32) {
-    set_last_error("precision %d is not CONTRACT_FP64 or CONTRACT_FP32", precision);
+  if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32 && precision != CONTRACT_FP32_SCALED) {
+    set_last_error("precision %d is not CONTRACT_FP64, CONTRACT_FP32 or CONTRACT_FP32_SCALED", precision);
     return CONTRACT_ERR_ARG;
   }
   const int chunk = chunk_of(g);
@@ -188,6 +188,11 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
       MesonGridReq &R = a.req[r];
       R.propag_i = q.propag_i;
       R.propag_j = q.propag_j;
+      if (precision == CONTRACT_FP32_SCALED) { // the site exponents follow the complex64 entries
+        const size_t V = (size_t)q.dims[0] * q.dims[1] * q.dims[2] * q.dims[3];
+        R.exp_i = static_cast<const signed char *>(q.propag_i) + V * SCALED_SITE_BYTES;
+        R.exp_j = static_cast<const signed char *>(q.propag_j) + V * SCALED_SITE_BYTES;
+      }
       Geometry geo;
       resolve_geometry(geo, q.dims, q.site_order, q.global, q.origin); // validated above
       for (int mu = 0; mu < 4; ++mu) R.dims[mu] = q.dims[mu];
@@ -220,7 +225,10 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
         f.out_plus[p] = m.plus;
         f.out_minus[p] = m.minus;
       }
-      launch_meson_grid(a, precision == CONTRACT_FP32, g.tile, grid, (size_t)tab * a.np, stream);
+      launch_meson_grid(a,
+                        precision == CONTRACT_FP32_SCALED ? GRID_FP32_SCALED
+                                                          : (precision == CONTRACT_FP32 ? GRID_FP32 : GRID_FP64),
+                        g.tile, grid, (size_t)tab * a.np, stream);
       CONTRACT_EXT_HIP(hipGetLastError());
       f.np = a.np;
       launch_meson_grid_finish(f, blocks, stream);

@@ -52,7 +52,7 @@ __device__ __forceinline__ int scale_block(float2 (&b)[9])
   return ex;
 }
 
-template <typename Real, int NS>
+template <typename Real, int NS, bool SCALED = false>
 __global__ __launch_bounds__(GRID_THREADS)
 __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) void meson_grid_kernel(
     MesonGridArgs a)
@@ -194,7 +194,12 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
         C xb[9], yb[9];
         load_block(xb, si, i, w);
         int ex = 0;
-        if constexpr (sizeof(Real) == 4) ex = scale_block(xb);
+        if constexpr (SCALED) { // site-scaled storage: the site's exponents (a site past the item reads the last one)
+          const size_t site = st + (size_t)ls < last ? st + (size_t)ls : last - 1;
+          ex = (int)R.exp_i[site] + (int)R.exp_j[site];
+        } else if constexpr (sizeof(Real) == 4) {
+          ex = scale_block(xb);
+        }
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
           load_block(yb, sj, j, l);
@@ -233,20 +238,24 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
   }
 }
 
-template <typename Real>
+template <typename Real, bool SCALED = false>
 static void launch_grid_t(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
 {
   if (tile_sites == 8)
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 8>), dim3((unsigned)blocks), dim3(GRID_THREADS), shmem, stream, a);
+    hipLaunchKernelGGL((meson_grid_kernel<Real, 8, SCALED>), dim3((unsigned)blocks), dim3(GRID_THREADS), shmem,
+                       stream, a);
   else
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 4>), dim3((unsigned)blocks), dim3(GRID_THREADS), shmem, stream, a);
+    hipLaunchKernelGGL((meson_grid_kernel<Real, 4, SCALED>), dim3((unsigned)blocks), dim3(GRID_THREADS), shmem,
+                       stream, a);
 }
 
-void launch_meson_grid(const MesonGridArgs &a, bool fp32, int tile_sites, size_t blocks, size_t tab_entries,
+void launch_meson_grid(const MesonGridArgs &a, int storage, int tile_sites, size_t blocks, size_t tab_entries,
                        hipStream_t stream)
 {
   const size_t shmem = tab_entries * sizeof(double2);
-  if (fp32)
+  if (storage == GRID_FP32_SCALED)
+    launch_grid_t<float, true>(a, tile_sites, blocks, shmem, stream);
+  else if (storage == GRID_FP32)
     launch_grid_t<float>(a, tile_sites, blocks, shmem, stream);
   else
     launch_grid_t<double>(a, tile_sites, blocks, shmem, stream);

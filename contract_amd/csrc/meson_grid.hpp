@@ -18,6 +18,7 @@ constexpr int GRID_PARTIAL = 16 * 2 * GRID_COLS; // doubles per work item: [row]
 
 struct MesonGridReq {
   const void *propag_i, *propag_j;
+  const signed char *exp_i, *exp_j; // site exponents (FP32 site-scaled storage only)
   int dims[4];
   int gdims[3], origin[3]; // phases exp(-2 pi i p (x + origin) / gdims)
   int order;            // 0 lexicographic, 1 even-odd
@@ -55,7 +56,8 @@ struct MesonGridFinish {
 
 // tile_sites: sites staged per step (4 or 8); blocks: persistent grid size;
 // tab_entries: np x max(Lx + Ly + Lz + 3), the per-momentum phase tables in LDS.
-void launch_meson_grid(const MesonGridArgs &a, bool fp32, int tile_sites, size_t blocks, size_t tab_entries,
+enum GridStorage { GRID_FP64 = 0, GRID_FP32 = 1, GRID_FP32_SCALED = 2 };
+void launch_meson_grid(const MesonGridArgs &a, int storage, int tile_sites, size_t blocks, size_t tab_entries,
                        hipStream_t stream);
 void launch_meson_grid_finish(const MesonGridFinish &f, int nblocks, hipStream_t stream);
 

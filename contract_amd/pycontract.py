@@ -145,6 +145,8 @@ class ScaledPropagator:
     `data` is a uint8 device tensor of V * 1153 bytes, complex64 P(x) 2^-e(x)
     followed by the int8 site exponents e(x)."""
 
+    dtype = "fp32_scaled"  # storage form tag (not a torch dtype)
+
     def __init__(self, data, volume: int):
         self.data, self.volume = data, int(volume)
         self.device = data.device
@@ -476,18 +478,13 @@ def mesonGridProject(requests, momenta, tuning=None, outs=None, workspace=None, 
     exp(-i p.x) meson_two_point(propag_i, propag_j; g1, g2)(x).
 
     All requests go through one persistent launch (per 32 requests and 8
-    momenta).  complex64 propagators select the FP32-contraction /
-    FP64-accumulation form; all requests must share the dtype."""
+    momenta).  complex64 propagators (or ScaledPropagator, see toScaledFP32)
+    select the FP32-contraction / FP64-accumulation form; all requests must
+    share the storage form."""
     import torch
     if not requests:
         return []
-    dt = requests[0][0].dtype
-    for req in requests:
-        if req[0].dtype != dt or req[1].dtype != dt:
-            raise TypeError("all propagators of a batch must have the same dtype")
-        _volume(req[0])
-        _volume(req[1])
-    precision = CONTRACT_FP32 if dt == torch.complex64 else CONTRACT_FP64
+    precision = _precision([p for req in requests for p in req[:2]])
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
     if outs is None:
         outs = [torch.empty((16, 16, mom.shape[0], int(req[2][3])), dtype=torch.complex128, device=req[0].device)

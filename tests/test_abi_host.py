@@ -187,7 +187,7 @@ def test_meson_grid_rejects_bad_arguments_with_codes():
                                              ctypes.byref(t) if t else None, None, 0, None)
 
     assert call(nreq=0) != 0 and "nreq" in contract_amd.last_error()
-    assert call(prec=2) != 0 and "precision" in contract_amd.last_error()
+    assert call(prec=3) != 0 and "precision" in contract_amd.last_error()
     assert call(nmom=0) != 0 and "nmom" in contract_amd.last_error()
     assert call(tuning=(0, 6, 0)) != 0 and "tile_sites" in contract_amd.last_error()
     assert call(tuning=(0, 0, 9)) != 0 and "blocks_per_cu" in contract_amd.last_error()

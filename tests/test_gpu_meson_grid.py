@@ -194,3 +194,33 @@ def test_sublattice_geometry(dev):
     got = pc.mesonGridProject([req], MOM7 + [(2, -1, 3)])[0].cpu().numpy()
     want = oracle.meson_grid_project(a, b, dims, MOM7 + [(2, -1, 3)], 0, G, o)
     assert grid_error(got, want) < 1e-12
+
+
+def scaled_batch(dev, reqs):
+    torch, pc, _ = dev
+    return [(pc.toScaledFP32(r[0].to(torch.complex128)), pc.toScaledFP32(r[1].to(torch.complex128)), *r[2:])
+            for r in reqs]
+
+
+@pytest.mark.parametrize("tuning", [(16, 4, 2), (8, 8, 2)])
+def test_fp32_site_scaled_batch(dev, tuning):
+    """Site-scaled storage of the same complex64 values: the mantissas are the
+    same (power-of-two scaling is exact) and FP32 rounding is scale-invariant,
+    so the result equals the plain FP32 form bit for bit; both within 1e-6 of
+    the FP64 oracle."""
+    torch, pc, _ = dev
+    reqs, hosts = uneven_batch(dev, np.complex64)
+    plain = [o.cpu().numpy() for o in pc.mesonGridProject(reqs, MOM7, tuning=tuning)]
+    scaled = [o.cpu().numpy() for o in pc.mesonGridProject(scaled_batch(dev, reqs), MOM7, tuning=tuning)]
+    for p, s, h in zip(plain, scaled, hosts):
+        assert np.array_equal(p, s)
+        assert timeslice_error(s, oracle_grid(h, MOM7)) < 1e-6
+
+
+def test_fp32_site_scaled_steep_decay(dev):
+    torch, pc, _ = dev
+    req, host = make_request(dev, (2, 2, 2, 64), 43, 0, alpha=0.8)
+    got = pc.mesonGridProject(scaled_batch(dev, [req]), MOM7[:3])[0].cpu().numpy()
+    want = oracle_grid(host, MOM7[:3])
+    assert np.abs(want[:, :, 0, -1]).max() < 1e-40
+    assert timeslice_error(got, want) < 1e-6
