@@ -21,22 +21,32 @@ namespace
 {
 
 struct GridTuning {
-  int chunk, tile, bpc;
+  int chunk, tile, bpc, unit; // unit: 1 vector-ALU traces, 2 matrix-core traces
 };
 
-GridTuning resolve_tuning(const contract_meson_grid_tuning *t)
+GridTuning resolve_tuning(const contract_meson_grid_tuning *t, int precision)
 {
-  GridTuning g{1024, 4, 2};
+  GridTuning g{1024, 4, 2, 0};
   if (t) {
     if (t->tile_sites) g.tile = t->tile_sites;
     if (t->sites_per_item) g.chunk = t->sites_per_item;
     if (t->blocks_per_cu) g.bpc = t->blocks_per_cu;
+    g.unit = t->trace_unit;
   }
+  if (g.unit == 0) g.unit = precision == CONTRACT_FP32 ? 1 : 2;
   return g;
 }
 
-int check_tuning(const GridTuning &g)
+int check_tuning(const GridTuning &g, int precision)
 {
+  if (g.unit != 1 && g.unit != 2) {
+    set_last_error("trace_unit = %d is not 0, 1 or 2", g.unit);
+    return CONTRACT_ERR_ARG;
+  }
+  if (g.unit == 2 && precision == CONTRACT_FP32) {
+    set_last_error("trace_unit = 2 (matrix cores) needs CONTRACT_FP64 or CONTRACT_FP32_SCALED storage");
+    return CONTRACT_ERR_ARG;
+  }
   if (g.tile != 4 && g.tile != 8) {
     set_last_error("tile_sites = %d is not 4 or 8", g.tile);
     return CONTRACT_ERR_ARG;
@@ -125,8 +135,9 @@ extern "C" {
 size_t contract_meson_grid_workspace_bytes(const contract_meson_request *reqs, int nreq, int nmom,
                                            const contract_meson_grid_tuning *tuning)
 {
-  const GridTuning g = resolve_tuning(tuning);
-  if (check_tuning(g) != CONTRACT_OK || check_requests(reqs, nreq) != CONTRACT_OK || nmom < 1 || nmom > MAX_MOM)
+  const GridTuning g = resolve_tuning(tuning, CONTRACT_FP64); // the unit does not change the workspace
+  if (check_tuning(g, CONTRACT_FP64) != CONTRACT_OK || check_requests(reqs, nreq) != CONTRACT_OK || nmom < 1 ||
+      nmom > MAX_MOM)
     return 0;
   return max_group_items(reqs, nreq, chunk_of(g)) * GRID_PARTIAL * sizeof(double);
 }
@@ -137,14 +148,14 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
 {
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   int rc;
-  const GridTuning g = resolve_tuning(tuning);
-  if ((rc = check_tuning(g)) != CONTRACT_OK) return rc;
-  if ((rc = check_requests(reqs, nreq)) != CONTRACT_OK) return rc;
-  if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
   if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32 && precision != CONTRACT_FP32_SCALED) {
     set_last_error("precision %d is not CONTRACT_FP64, CONTRACT_FP32 or CONTRACT_FP32_SCALED", precision);
     return CONTRACT_ERR_ARG;
   }
+  const GridTuning g = resolve_tuning(tuning, precision);
+  if ((rc = check_tuning(g, precision)) != CONTRACT_OK) return rc;
+  if ((rc = check_requests(reqs, nreq)) != CONTRACT_OK) return rc;
+  if ((rc = check_momenta(momenta, nmom)) != CONTRACT_OK) return rc;
   const int chunk = chunk_of(g);
   const size_t group_items = max_group_items(reqs, nreq, chunk);
   if (group_items >= ((size_t)1 << 32)) {
@@ -228,7 +239,7 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
       launch_meson_grid(a,
                         precision == CONTRACT_FP32_SCALED ? GRID_FP32_SCALED
                                                           : (precision == CONTRACT_FP32 ? GRID_FP32 : GRID_FP64),
-                        g.tile, grid, (size_t)tab * a.np, stream);
+                        g.unit == 2, g.tile, grid, (size_t)tab * a.np, stream);
       CONTRACT_EXT_HIP(hipGetLastError());
       f.np = a.np;
       launch_meson_grid_finish(f, blocks, stream);

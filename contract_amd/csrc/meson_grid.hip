@@ -22,12 +22,15 @@
 // persistent grid walks the flat item space of the whole batch, so uneven
 // requests balance across the chip.
 
+#include <type_traits>
+
 #include "meson_grid.hpp"
 
 namespace contract_amd
 {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int nonneg_mod(int a, int L)
 {
@@ -52,9 +55,19 @@ __device__ __forceinline__ int scale_block(float2 (&b)[9])
   return ex;
 }
 
-template <typename Real, int NS, bool SCALED = false>
+// Per-site traces K[ik][jl] = sum_c P_i[ik][c] conj(P_j[jl][c]) (c = the
+// nine colour pairs) are a complex 16 x 9 x 16 product per site.  Two forms:
+//  KMFMA = false: on the vector ALUs, lane (i, j) of wave k forms K[i k][j l]
+//    for l = 0..3 from one P_i block and four P_j blocks read from LDS;
+//  KMFMA = true: on the matrix cores, wave w forms all of K for site w of each
+//    four-site step with 12 v_mfma 16x16x4 (3 colour chunks x re/im pairs;
+//    A = P_i rows, B = P_j rows), each staged entry read from LDS once; K goes
+//    through LDS once more to become the projection's B operand (4 sites x 16
+//    columns).  Needs FP64 or site-scaled FP32 storage: plain FP32 products of
+//    two steeply decaying propagators could underflow without a per-row scale.
+template <typename Real, int NS, bool SCALED = false, bool KMFMA = false>
 __global__ __launch_bounds__(GRID_THREADS)
-__attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) void meson_grid_kernel(
+__attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 : 2))) void meson_grid_kernel(
     MesonGridArgs a)
 {
   static_assert(NS % SITES_PER_WAVE == 0, "tile is a whole number of MFMA site steps");
@@ -65,6 +78,10 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
   constexpr int PER_PROP = (CH + GRID_THREADS - 1) / GRID_THREADS; // chunks per thread and propagator
   constexpr int TILE16 = NS * T::LDS16;          // one propagator tile in LDS, 16-byte units
   __shared__ C lds[2][NS * SL];                  // [propagator]
+  // K of a four-site step, [slot][site][ik * 16 + jl]; two slots, so one
+  // barrier per step orders the writes of step n + 1 after the reads of step n
+  __shared__ double2 kbuf[KMFMA ? 2 : 1][KMFMA ? SITES_PER_WAVE : 1][256];
+  int kslot = 0;
   extern __shared__ double2 tab[];
   const int tid = threadIdx.x;
   const int w = tid >> 6, lane = tid & 63;
@@ -136,7 +153,9 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
     }
     const int pbase = par + tsl;
 
-    dbl4 acc_re[4], acc_im[4]; // column tiles l = 0..3 of k = w; B = K re / K im
+    // column tiles of k = w: l = 0..3 (vector traces, lane q = (i, j)) or
+    // i = 0..3 (matrix-core traces, lane q = (j, l)); B = K re / K im
+    dbl4 acc_re[4], acc_im[4];
 #pragma unroll
     for (int l = 0; l < 4; ++l) acc_re[l] = acc_im[l] = dbl4{0.0, 0.0, 0.0, 0.0};
 
@@ -190,6 +209,58 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
           const double2 ph = cmul(cmul(tx[x], ty[y]), tz[z]);
           av = (q & 1) ? ph.y : ph.x;
         }
+        if constexpr (KMFMA) {
+          using KAcc = typename std::conditional<sizeof(Real) == 8, dbl4, flt4>::type;
+          const int sk = h * SITES_PER_WAVE + w; // this wave's site
+          const C *xs = lds[0] + sk * SL, *ys = lds[1] + sk * SL;
+          KAcc kr = KAcc{0, 0, 0, 0}, ki = KAcc{0, 0, 0, 0};
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            // A[ik = q][kk = g] = P_i[q][c], B[kk = g][jl = q] = P_j[q][c], c = 4 ch + g
+            const int cc = 4 * ch + g;
+            const int cl = cc < 9 ? cc : 8;
+            C xv = xs[q * 9 + cl], yv = ys[q * 9 + cl];
+            if (cc >= 9) xv = yv = czero<C>();
+            if constexpr (sizeof(Real) == 8) {
+              kr = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.x, yv.x, kr, 0, 0, 0);
+              kr = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.y, yv.y, kr, 0, 0, 0);
+              ki = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.y, yv.x, ki, 0, 0, 0);
+              ki = __builtin_amdgcn_mfma_f64_16x16x4f64(-xv.x, yv.y, ki, 0, 0, 0);
+            } else {
+              kr = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, yv.x, kr, 0, 0, 0);
+              kr = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, yv.y, kr, 0, 0, 0);
+              ki = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, yv.x, ki, 0, 0, 0);
+              ki = __builtin_amdgcn_mfma_f32_16x16x4f32(-xv.x, yv.y, ki, 0, 0, 0);
+            }
+          }
+          int ex = 0;
+          if constexpr (SCALED) {
+            const size_t site = st + (size_t)sk < last ? st + (size_t)sk : last - 1;
+            ex = (int)R.exp_i[site] + (int)R.exp_j[site];
+          }
+          // 16x16x4 output: lane (g, q) register v holds D[g + 4v][q] (f64) or
+          // D[4g + v][q] (f32)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = sizeof(Real) == 8 ? g + 4 * v : 4 * g + v;
+            double re = (double)kr[v], im = (double)ki[v];
+            if constexpr (SCALED) {
+              re = ldexp(re, ex);
+              im = ldexp(im, ex);
+            }
+            kbuf[kslot][w][row * 16 + q] = make_double2(re, im);
+          }
+          __syncthreads();
+          // rows ik = 4 i + k of this wave's k = w; B[site g][jl = q]
+#pragma unroll
+          for (int i2 = 0; i2 < 4; ++i2) {
+            const double2 kv = kbuf[kslot][g][(4 * i2 + w) * 16 + q];
+            acc_re[i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, kv.x, acc_re[i2], 0, 0, 0);
+            acc_im[i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, kv.y, acc_im[i2], 0, 0, 0);
+          }
+          kslot ^= 1;
+          continue;
+        }
         const C *si = lds[0] + ls * SL, *sj = lds[1] + ls * SL;
         C xb[9], yb[9];
         load_block(xb, si, i, w);
@@ -224,41 +295,49 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8) ? 1 : 2))) voi
         }
       }
     }
-    // v_mfma_f64_16x16x4f64: lane (g, q) register v holds D[g + 4v][q]
+    // v_mfma_f64_16x16x4f64: lane (g, q) register v holds D[g + 4v][q];
+    // partial column (k l) * 16 + (i j) with k = w
     double *dst = a.partial + item * (size_t)GRID_PARTIAL;
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = g + 4 * v;
-        const int col = (w * 4 + l) * 16 + q;
-        dst[(row * 2 + 0) * GRID_COLS + col] = acc_re[l][v];
-        dst[(row * 2 + 1) * GRID_COLS + col] = acc_im[l][v];
+        const int col = KMFMA ? (w * 4 + (q & 3)) * 16 + t * 4 + (q >> 2) : (w * 4 + t) * 16 + q;
+        dst[(row * 2 + 0) * GRID_COLS + col] = acc_re[t][v];
+        dst[(row * 2 + 1) * GRID_COLS + col] = acc_im[t][v];
       }
   }
 }
 
-template <typename Real, bool SCALED = false>
+template <typename Real, bool SCALED, bool KMFMA>
 static void launch_grid_t(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
 {
   if (tile_sites == 8)
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 8, SCALED>), dim3((unsigned)blocks), dim3(GRID_THREADS), shmem,
-                       stream, a);
+    hipLaunchKernelGGL((meson_grid_kernel<Real, 8, SCALED, KMFMA>), dim3((unsigned)blocks), dim3(GRID_THREADS),
+                       shmem, stream, a);
   else
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 4, SCALED>), dim3((unsigned)blocks), dim3(GRID_THREADS), shmem,
-                       stream, a);
+    hipLaunchKernelGGL((meson_grid_kernel<Real, 4, SCALED, KMFMA>), dim3((unsigned)blocks), dim3(GRID_THREADS),
+                       shmem, stream, a);
 }
 
-void launch_meson_grid(const MesonGridArgs &a, int storage, int tile_sites, size_t blocks, size_t tab_entries,
-                       hipStream_t stream)
+void launch_meson_grid(const MesonGridArgs &a, int storage, bool mfma_traces, int tile_sites, size_t blocks,
+                       size_t tab_entries, hipStream_t stream)
 {
   const size_t shmem = tab_entries * sizeof(double2);
-  if (storage == GRID_FP32_SCALED)
-    launch_grid_t<float, true>(a, tile_sites, blocks, shmem, stream);
-  else if (storage == GRID_FP32)
-    launch_grid_t<float>(a, tile_sites, blocks, shmem, stream);
-  else
-    launch_grid_t<double>(a, tile_sites, blocks, shmem, stream);
+  if (storage == GRID_FP32_SCALED) {
+    if (mfma_traces)
+      launch_grid_t<float, true, true>(a, tile_sites, blocks, shmem, stream);
+    else
+      launch_grid_t<float, true, false>(a, tile_sites, blocks, shmem, stream);
+  } else if (storage == GRID_FP32) {
+    launch_grid_t<float, false, false>(a, tile_sites, blocks, shmem, stream);
+  } else {
+    if (mfma_traces)
+      launch_grid_t<double, false, true>(a, tile_sites, blocks, shmem, stream);
+    else
+      launch_grid_t<double, false, false>(a, tile_sites, blocks, shmem, stream);
+  }
 }
 
 __device__ __forceinline__ void nsum(double &s, double &c, double v)

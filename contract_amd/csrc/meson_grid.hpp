@@ -57,8 +57,8 @@ struct MesonGridFinish {
 // tile_sites: sites staged per step (4 or 8); blocks: persistent grid size;
 // tab_entries: np x max(Lx + Ly + Lz + 3), the per-momentum phase tables in LDS.
 enum GridStorage { GRID_FP64 = 0, GRID_FP32 = 1, GRID_FP32_SCALED = 2 };
-void launch_meson_grid(const MesonGridArgs &a, int storage, int tile_sites, size_t blocks, size_t tab_entries,
-                       hipStream_t stream);
+void launch_meson_grid(const MesonGridArgs &a, int storage, bool mfma_traces, int tile_sites, size_t blocks,
+                       size_t tab_entries, hipStream_t stream);
 void launch_meson_grid_finish(const MesonGridFinish &f, int nblocks, hipStream_t stream);
 
 } // namespace contract_amd

@@ -434,11 +434,13 @@ def _fill_geometry(r, req):
 
 
 class MesonGridTuning(ctypes.Structure):
-    """contract_meson_grid_tuning; zero fields take the library default."""
-    _fields_ = [("sites_per_item", ctypes.c_int), ("tile_sites", ctypes.c_int), ("blocks_per_cu", ctypes.c_int)]
+    """contract_meson_grid_tuning; zero fields take the library default.
+    trace_unit: 1 vector ALUs, 2 matrix cores (FP64 / site-scaled FP32)."""
+    _fields_ = [("sites_per_item", ctypes.c_int), ("tile_sites", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
+                ("trace_unit", ctypes.c_int)]
 
     def key(self):
-        return (self.sites_per_item, self.tile_sites, self.blocks_per_cu)
+        return (self.sites_per_item, self.tile_sites, self.blocks_per_cu, self.trace_unit)
 
 
 def _tuning(tuning):

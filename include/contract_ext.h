@@ -151,6 +151,9 @@ typedef struct {
   int sites_per_item; /* sites per work item (rounded up to the tile) */
   int tile_sites;     /* sites staged per step: 4 or 8 */
   int blocks_per_cu;  /* persistent workgroups per compute unit: 1..8 */
+  int trace_unit;     /* per-site traces on 1: the vector ALUs, 2: the matrix
+                         cores (FP64 and FP32 site-scaled storage only);
+                         0: the library's choice */
 } contract_meson_grid_tuning;
 
 size_t contract_meson_grid_workspace_bytes(const contract_meson_request *reqs, int nreq, int nmom,

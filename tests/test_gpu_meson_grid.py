@@ -202,12 +202,12 @@ def scaled_batch(dev, reqs):
             for r in reqs]
 
 
-@pytest.mark.parametrize("tuning", [(16, 4, 2), (8, 8, 2)])
+@pytest.mark.parametrize("tuning", [(16, 4, 2, 1), (8, 8, 2, 1)])
 def test_fp32_site_scaled_batch(dev, tuning):
-    """Site-scaled storage of the same complex64 values: the mantissas are the
-    same (power-of-two scaling is exact) and FP32 rounding is scale-invariant,
-    so the result equals the plain FP32 form bit for bit; both within 1e-6 of
-    the FP64 oracle."""
+    """Site-scaled storage of the same complex64 values, traces on the vector
+    ALUs in both: the mantissas are the same (power-of-two scaling is exact)
+    and FP32 rounding is scale-invariant, so the result equals the plain FP32
+    form bit for bit; both within 1e-6 of the FP64 oracle."""
     torch, pc, _ = dev
     reqs, hosts = uneven_batch(dev, np.complex64)
     plain = [o.cpu().numpy() for o in pc.mesonGridProject(reqs, MOM7, tuning=tuning)]
@@ -224,3 +224,29 @@ def test_fp32_site_scaled_steep_decay(dev):
     want = oracle_grid(host, MOM7[:3])
     assert np.abs(want[:, :, 0, -1]).max() < 1e-40
     assert timeslice_error(got, want) < 1e-6
+
+
+@pytest.mark.parametrize("unit", [1, 2])
+@pytest.mark.parametrize("tile", [4, 8])
+def test_trace_units_match_oracle(dev, unit, tile):
+    """Per-site traces on the vector ALUs (1) or the matrix cores (2), both
+    tiles, FP64 and site-scaled FP32, ragged items (27-site slices)."""
+    torch, pc, _ = dev
+    reqs, hosts = uneven_batch(dev)
+    outs = pc.mesonGridProject(reqs, MOM7, tuning=(16, tile, 2, unit))
+    for out, h in zip(outs, hosts):
+        assert grid_error(out.cpu().numpy(), oracle_grid(h, MOM7)) < 1e-12
+    r, h = make_request(dev, (3, 3, 3, 4), 77, 1)
+    got = pc.mesonGridProject([r], MOM27, tuning=(20, tile, 1, unit))[0].cpu().numpy()
+    assert grid_error(got, oracle_grid(h, MOM27)) < 1e-12
+    f32, h32 = uneven_batch(dev, np.complex64)
+    outs = pc.mesonGridProject(scaled_batch(dev, f32), MOM7, tuning=(16, tile, 2, unit))
+    for out, h in zip(outs, h32):
+        assert timeslice_error(out.cpu().numpy(), oracle_grid(h, MOM7)) < 1e-6
+
+
+def test_matrix_core_traces_need_fp64_or_scaled(dev):
+    torch, pc, _ = dev
+    reqs, _ = uneven_batch(dev, np.complex64)
+    with pytest.raises(RuntimeError, match="trace_unit"):
+        pc.mesonGridProject(reqs, MOM7, tuning=(16, 4, 2, 2))
