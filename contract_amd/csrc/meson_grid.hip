@@ -60,8 +60,9 @@ __device__ __forceinline__ int scale_block(float2 (&b)[9])
 //  KMFMA = false: on the vector ALUs, lane (i, j) of wave k forms K[i k][j l]
 //    for l = 0..3 from one P_i block and four P_j blocks read from LDS;
 //  KMFMA = true: on the matrix cores, wave w forms all of K for site w of each
-//    four-site step with 12 v_mfma 16x16x4 (3 colour chunks x re/im pairs;
-//    A = P_i rows, B = P_j rows), each staged entry read from LDS once; K goes
+//    four-site step with 9 (FP64) or 12 (FP32) v_mfma 16x16x4 (3 colour
+//    chunks x 3 or 4 real products; A = P_i rows, B = P_j rows), each staged
+//    entry read from LDS once; K goes
 //    through LDS once more to become the projection's B operand (4 sites x 16
 //    columns).  Needs FP64 or site-scaled FP32 storage: plain FP32 products of
 //    two steeply decaying propagators could underflow without a per-row scale.
@@ -78,9 +79,13 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
   constexpr int PER_PROP = (CH + GRID_THREADS - 1) / GRID_THREADS; // chunks per thread and propagator
   constexpr int TILE16 = NS * T::LDS16;          // one propagator tile in LDS, 16-byte units
   __shared__ C lds[2][NS * SL];                  // [propagator]
-  // K of a four-site step, [slot][site][ik * 16 + jl]; two slots, so one
-  // barrier per step orders the writes of step n + 1 after the reads of step n
-  __shared__ double2 kbuf[KMFMA ? 2 : 1][KMFMA ? SITES_PER_WAVE : 1][256];
+  // K of a four-site step, [slot][site][ik][jl]; two slots, so one barrier
+  // per step orders the writes of step n + 1 after the reads of step n.  Rows
+  // padded to 17 entries and sites to 16 * 17 + 1, so neither the MFMA-layout
+  // writes (lane groups on rows g + 4v) nor the B-operand reads (lane groups
+  // on sites) of a wave fall on one LDS bank
+  constexpr int KROW = 17, KSITE = 16 * KROW + 1;
+  __shared__ double2 kbuf[KMFMA ? 2 : 1][KMFMA ? SITES_PER_WAVE : 1][KSITE];
   int kslot = 0;
   extern __shared__ double2 tab[];
   const int tid = threadIdx.x;
@@ -213,7 +218,13 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
           using KAcc = typename std::conditional<sizeof(Real) == 8, dbl4, flt4>::type;
           const int sk = h * SITES_PER_WAVE + w; // this wave's site
           const C *xs = lds[0] + sk * SL, *ys = lds[1] + sk * SL;
-          KAcc kr = KAcc{0, 0, 0, 0}, ki = KAcc{0, 0, 0, 0};
+          // FP64: three real products per colour chunk (Gauss), T1 = Xr Yr^T,
+          // T2 = Xi Yi^T, T3 = (Xr + Xi)(Yr - Yi)^T, so K = X conj(Y)^T has
+          // Re = T1 + T2, Im = T3 - T1 + T2 (9 FP64 MFMAs per site, -4% on the
+          // batch: FP64 matrix issue is the scarcer resource).  FP32: four
+          // products (FP32 MFMA issue is twice as fast, and the Gauss form's
+          // cancellation in Im costs FP32 accuracy)
+          KAcc kr = KAcc{0, 0, 0, 0}, ki = KAcc{0, 0, 0, 0}, t3 = KAcc{0, 0, 0, 0};
 #pragma unroll
           for (int ch = 0; ch < 3; ++ch) {
             // A[ik = q][kk = g] = P_i[q][c], B[kk = g][jl = q] = P_j[q][c], c = 4 ch + g
@@ -222,16 +233,20 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
             C xv = xs[q * 9 + cl], yv = ys[q * 9 + cl];
             if (cc >= 9) xv = yv = czero<C>();
             if constexpr (sizeof(Real) == 8) {
-              kr = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.x, yv.x, kr, 0, 0, 0);
-              kr = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.y, yv.y, kr, 0, 0, 0);
-              ki = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.y, yv.x, ki, 0, 0, 0);
-              ki = __builtin_amdgcn_mfma_f64_16x16x4f64(-xv.x, yv.y, ki, 0, 0, 0);
+              kr = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.x, yv.x, kr, 0, 0, 0); // T1
+              ki = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.y, yv.y, ki, 0, 0, 0); // T2
+              t3 = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.x + xv.y, yv.x - yv.y, t3, 0, 0, 0);
             } else {
               kr = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, yv.x, kr, 0, 0, 0);
               kr = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, yv.y, kr, 0, 0, 0);
               ki = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, yv.x, ki, 0, 0, 0);
               ki = __builtin_amdgcn_mfma_f32_16x16x4f32(-xv.x, yv.y, ki, 0, 0, 0);
             }
+          }
+          if constexpr (sizeof(Real) == 8) {
+            const KAcc t1 = kr, t2 = ki;
+            kr = t1 + t2;
+            ki = t3 - t1 + t2;
           }
           int ex = 0;
           if constexpr (SCALED) {
@@ -248,13 +263,13 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
               re = ldexp(re, ex);
               im = ldexp(im, ex);
             }
-            kbuf[kslot][w][row * 16 + q] = make_double2(re, im);
+            kbuf[kslot][w][row * KROW + q] = make_double2(re, im);
           }
           __syncthreads();
           // rows ik = 4 i + k of this wave's k = w; B[site g][jl = q]
 #pragma unroll
           for (int i2 = 0; i2 < 4; ++i2) {
-            const double2 kv = kbuf[kslot][g][(4 * i2 + w) * 16 + q];
+            const double2 kv = kbuf[kslot][g][(4 * i2 + w) * KROW + q];
             acc_re[i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, kv.x, acc_re[i2], 0, 0, 0);
             acc_im[i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, kv.y, acc_im[i2], 0, 0, 0);
           }
