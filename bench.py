@@ -243,12 +243,24 @@ def baryon_batch(torch, pc, dev, steps, warmup, hip=None, timer=None, nreq=16, d
 
     wall_b, kern_b = time_steps(torch, batched, steps, warmup, hip, timer)
     got = [o.cpu() for o in outs]
+    # the same batch captured once in a HIP graph (it allocates nothing with a
+    # caller workspace) and replayed
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=side):
+        pc.baryonTwoPointProjectBatch(reqs, TERMS, GAMMA_CG5, GAMMA_CG5, gammas, MOM27, outs=outs, workspace=ws,
+                                      stream=side.cuda_stream)
+    wall_g, _ = time_steps(torch, graph.replay, steps, warmup)
+    same_graph = all(torch.equal(a, o.cpu()) for a, o in zip(got, outs))
+    del graph
     wall_s, _ = time_steps(torch, singles, steps, warmup)
     same = all(torch.equal(a, o.cpu()) for a, o in zip(got, outs))
     gbs = BYTES_PER_SITE_FP64 * V * nreq / (wall_b * 1e-3) / 1e9
     return {"workload": f"proton 2pt for {nreq} source positions, {dims[0]}^3x{dims[3]} each, Cg5, 16 insertions, "
                         "27 momenta",
-            "requests": nreq, "batched_ms": wall_b, "batched_kernel_ms": kern_b,
+            "requests": nreq, "batched_ms": wall_b, "batched_kernel_ms": kern_b, "batched_graph_ms": wall_g,
+            "graph_replay_bit_identical": same_graph,
             "kernel_GBs": None if kern_b is None else BYTES_PER_SITE_FP64 * V * nreq / (kern_b * 1e-3) / 1e9,
             "singles_ms": wall_s, "speedup": wall_s / wall_b,
             "sites_per_s": V * nreq / (wall_b * 1e-3), "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
