@@ -370,6 +370,7 @@ __device__ __forceinline__ void nsum(double &s, double &c, double v)
 // momentum +k (A + iB) and of -k (A - iB); thread (g1, g2) applies the gamma
 // pair to each requested sign.  Output out[g1][g2][momentum][t].
 constexpr int FINISH_GROUPS = 2;
+constexpr int FINISH_UNROLL = 4;
 
 __global__ __launch_bounds__(FINISH_GROUPS * GRID_COLS) void meson_grid_finish_kernel(MesonGridFinish f)
 {
@@ -385,11 +386,24 @@ __global__ __launch_bounds__(FINISH_GROUPS * GRID_COLS) void meson_grid_finish_k
   double s[4] = {0, 0, 0, 0}, cc[4] = {0, 0, 0, 0};
   for (int par = 0; par < nslices; ++par) {
     const size_t slice = (size_t)par * R.lt + t;
-    for (int ch = cg; ch < R.chunks_per_slice; ch += FINISH_GROUPS) {
-      const double *src = f.partial + (R.item0 + slice * R.chunks_per_slice + ch) * (size_t)GRID_PARTIAL;
-      // row 2p: cos weight, row 2p + 1: -sin weight; each with K re / K im
+    const double *base = f.partial + (R.item0 + slice * R.chunks_per_slice) * (size_t)GRID_PARTIAL +
+                         (size_t)(4 * p) * GRID_COLS + col;
+    // FINISH_UNROLL chunks' loads in flight at once, summed in chunk order
+    for (int ch0 = cg; ch0 < R.chunks_per_slice; ch0 += FINISH_GROUPS * FINISH_UNROLL) {
+      double v[FINISH_UNROLL][4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) nsum(s[u], cc[u], src[(4 * p + u) * GRID_COLS + col]);
+      for (int k = 0; k < FINISH_UNROLL; ++k) {
+        const int ch = ch0 + k * FINISH_GROUPS;
+        const double *src = base + (size_t)(ch < R.chunks_per_slice ? ch : ch0) * GRID_PARTIAL;
+        // row 2p: cos weight, row 2p + 1: -sin weight; each with K re / K im
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[k][u] = src[u * GRID_COLS];
+      }
+#pragma unroll
+      for (int k = 0; k < FINISH_UNROLL; ++k)
+        if (ch0 + k * FINISH_GROUPS < R.chunks_per_slice)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) nsum(s[u], cc[u], v[k][u]);
     }
   }
 #pragma unroll

@@ -254,15 +254,21 @@ class Engine:
     def _materialise(self, key, recycled=None):
         torch = self.torch
         with torch.cuda.stream(self.load_stream):
-            reuse = recycled.tensor if recycled is not None else None
-            if reuse is not None and tuple(reuse.shape) == (int(np.prod(key.dims)), 4, 4, 3, 3):
-                self.load_stream.wait_event(recycled.ready)
-            else:
-                reuse = None
+            cdt = torch.complex128 if key.dtype == "complex128" else torch.complex64
+            reuse = self._reuse(recycled, (int(np.prod(key.dims)), 4, 4, 3, 3), cdt, self.load_stream)
             t = device_propagator(torch, key, self.device, out=reuse)
             ev = torch.cuda.Event()
             ev.record(self.load_stream)
         return Entry(t, ev)
+
+    def _reuse(self, recycled, shape, dtype, stream):
+        """The memory of an evicted entry of the same byte size as a tensor of
+        `shape` and `dtype`, ordered after its last writer; None if there is
+        none."""
+        if recycled is None or recycled.tensor is None:
+            return None
+        stream.wait_event(recycled.ready)
+        return recycled.tensor.reshape(-1).view(self.torch.uint8).view(dtype).view(shape)
 
     def _index(self, values):
         """Small index tensor on the device, copied from pinned host memory
@@ -284,10 +290,27 @@ class Engine:
         return ts
 
     # -- launches --------------------------------------------------------------
+    @staticmethod
+    def _momentum_groups(union: Sequence[Tuple[int, int, int]]) -> List[List[Tuple[int, int, int]]]:
+        """The union of momenta cut into groups of at most 8 slots, a slot being
+        a momentum together with its negative when both are wanted (the grid
+        kernel accumulates k and -k as one cos/sin pair), so that one grid pass
+        (8 slots) reads every propagator once for up to 16 momenta."""
+        have = set(union)
+        slots, seen = [], set()
+        for m in union:
+            if m in seen:
+                continue
+            neg = (-m[0], -m[1], -m[2])
+            slot = [m] if neg == m or neg not in have else [m, neg]
+            seen.update(slot)
+            slots.append(slot)
+        return [[m for s in slots[g:g + 8] for m in s] for g in range(0, len(slots), 8)]
+
     def _launch_mesons(self, reqs: List[Request], stream) -> list:
-        """One batched grid launch per group of 8 momenta of the union of the
-        requests' momenta, each over the requests that need one of them; then
-        every request gathers its (gamma pair, momentum) entries."""
+        """One batched grid launch per group of 8 momentum slots of the union
+        of the requests' momenta, each over the requests that need one of
+        them; then every request gathers its (gamma pair, momentum) entries."""
         torch = self.torch
         keep = []
         union = sorted({tuple(m) for r in reqs for m in r.momenta})
@@ -301,8 +324,7 @@ class Engine:
             for dt, group in by_dtype.items():
                 tuning = self.grid_tuning[dt]
                 inputs = {r.rid: self._inputs(r, stream) for r in group}
-                for g0 in range(0, len(union), 8):
-                    moms = union[g0:g0 + 8]
+                for moms in self._momentum_groups(union):
                     sub = [r for r in group if any(tuple(m) in moms for m in r.momenta)]
                     if not sub:
                         continue
@@ -360,12 +382,16 @@ class Engine:
         key = r.out_key()
         with torch.cuda.stream(stream):
             pi, pj = self._inputs(r, stream)
-            spare = self._spare.pop(r.rid, None)
-            if spare is not None and spare.tensor is not None and spare.tensor.shape == pi.shape:
-                stream.wait_event(spare.ready)
-                out = spare.tensor
-            else:
-                out = torch.empty_like(pi)
+            out = self._reuse(self._spare.pop(r.rid, None), pi.shape, pi.dtype, stream)
+            if out is None:
+                # pool memory is allocated on the load stream only: torch's
+                # caching allocator reuses a freed block only for allocations
+                # on the stream it was allocated on, so evicted entries'
+                # memory then serves the next entries (a fresh multi-GB
+                # allocation costs the host tens of milliseconds).  Entries
+                # are evicted only after their last reader has finished.
+                with torch.cuda.stream(self.load_stream):
+                    out = torch.empty_like(pi)
             (gij, gkl), = r.gammas[:1]
             with pc.abi_stream(stream):
                 pc.baryon_diquark(out, pi, pj, int(np.prod(r.dims)), int(gij), int(gkl))
