@@ -513,7 +513,12 @@ template <typename Real, int NT> constexpr bool project_persistent()
 // of 241 VGPRs, 3-4% faster at 32^3x64.
 enum { TERMS_RUNTIME = 0, TERMS_NUCLEON = 1 };
 
-template <int NT, typename Real, int GIJ = -1, bool SCALED = false, int TERMS = TERMS_RUNTIME>
+// SHARE: every request of the launch has propag_n == propag_i (a nucleon's
+// two same-flavour quarks from one propagator, (u, d, u)).  That propagator
+// is read from HBM and staged once per site and serves as both P_i and P_n:
+// 4608 instead of 6912 bytes per site in FP64.  The tails' scratch is then
+// P_j's consumed staging area, since P_i's is still being read as P_n.
+template <int NT, typename Real, int GIJ = -1, bool SCALED = false, int TERMS = TERMS_RUNTIME, bool SHARE = false>
 __global__ __launch_bounds__(WAVE)
 __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : (NT == 1 ? CONTRACT_WPE32 : 1)))) void
 baryon_project_kernel(
@@ -522,7 +527,8 @@ baryon_project_kernel(
   using C = cplx<Real>;
   constexpr int SL = site_lds<C>();
   constexpr bool PERSIST = project_persistent<Real, NT>();
-  __shared__ C lds[3][SITES_PER_WAVE * SL];
+  constexpr int NP = SHARE ? 2 : 3; // distinct propagators staged per site
+  __shared__ C lds[NP][SITES_PER_WAVE * SL];
   extern __shared__ double2 tab[]; // Gx + Gy + Gz phase entries, sized at launch
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
@@ -543,16 +549,17 @@ baryon_project_kernel(
   // tile at st (padded lanes repeat the last site; their R is zero)
   auto site_exp = [&](const ProjectReq &rq, size_t st, int n) {
     const size_t k = st + (size_t)(g < n ? g : n - 1);
+    if constexpr (SHARE) return 2 * (int)rq.exp_i[k] + (int)rq.exp_j[k];
     return (int)rq.exp_i[k] + (int)rq.exp_j[k] + (int)rq.exp_n[k];
   };
-  Tile<C> buf[3];
+  Tile<C> buf[NP];
   int ex_buf = 0; // exponent sum of the tile in buf
   {
     const ProjectReq &pa = args.req[r];
     const int n = tile_sites(first, last);
     buf[0].load((const C *)pa.propag_i, first, n, lane);
     buf[1].load((const C *)pa.propag_j, first, n, lane);
-    buf[2].load((const C *)pa.propag_n, first, n, lane);
+    if constexpr (!SHARE) buf[NP - 1].load((const C *)pa.propag_n, first, n, lane);
     if constexpr (SCALED) ex_buf = site_exp(pa, first, n);
   }
 
@@ -581,7 +588,7 @@ baryon_project_kernel(
 
     for (size_t s = first; s < last; s += SITES_PER_WAVE) {
 #pragma unroll
-      for (int p = 0; p < 3; ++p) buf[p].store(lds[p], lane);
+      for (int p = 0; p < NP; ++p) buf[p].store(lds[p], lane);
       __syncthreads();
       int ex = ex_buf;
       const size_t next = s + SITES_PER_WAVE;
@@ -591,16 +598,19 @@ baryon_project_kernel(
         const int n = here ? tile_sites(next, last) : tile_sites(first_next, last_next);
         buf[0].load(here ? Pi : Qi, st, n, lane);
         buf[1].load(here ? Pj : Qj, st, n, lane);
-        buf[2].load(here ? Pn : Qn, st, n, lane);
+        if constexpr (!SHARE) buf[NP - 1].load(here ? Pn : Qn, st, n, lane);
         if constexpr (SCALED) ex_buf = site_exp(here ? pa : args.req[r_next], st, n);
       }
 
-      C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[2] + g * SL;
+      C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[SHARE ? 0 : 2] + g * SL;
+      C *scr = SHARE ? sj : si; // consumed once D is formed
       if constexpr (sizeof(Real) == 4 && !SCALED) {
         // P_i and P_j only: D of the rescaled blocks is O(1), so the tails'
         // products with P_n are as large as P_n's own entries and cannot
-        // underflow unless a stored entry already is subnormal
-        ex = normalise_site(si, q) + normalise_site(sj, q);
+        // underflow unless a stored entry already is subnormal.  A shared
+        // P_n = P_i is read rescaled, which its exponent counts twice.
+        const int ei = normalise_site(si, q);
+        ex = (SHARE ? 2 * ei : ei) + normalise_site(sj, q);
         __syncthreads();
       }
       C D[9];
@@ -610,13 +620,13 @@ baryon_project_kernel(
         diquark_tensor(D, si, sj, q, a.gij);
       double2 R = make_double2(0.0, 0.0);
       if constexpr (TERMS == TERMS_NUCLEON) { // types {IK_JL_NM, IM_JL_NK}, in that order
-        const C r0 = baryon_tail<0, C>(D, sn, si, lane, kl);
+        const C r0 = baryon_tail<0, C>(D, sn, scr, lane, kl);
         R = cfma(R, make_double2(a.w_re[0], a.w_im[0]), make_double2((double)r0.x, (double)r0.y));
-        const C r5 = baryon_tail<5, C>(D, sn, si, lane, kl);
+        const C r5 = baryon_tail<5, C>(D, sn, scr, lane, kl);
         R = cfma(R, make_double2(a.w_re[1], a.w_im[1]), make_double2((double)r5.x, (double)r5.y));
       } else {
         for (int t = 0; t < a.nterm; ++t) {
-          const C rt = baryon_tail_rt<C>(a.types[t], D, sn, si, lane, kl);
+          const C rt = baryon_tail_rt<C>(a.types[t], D, sn, scr, lane, kl);
           R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)rt.x, (double)rt.y));
         }
       }
@@ -637,11 +647,11 @@ baryon_project_kernel(
   }
 }
 
-template <int NT, typename Real, int GIJ, bool SCALED, int TERMS>
+template <int NT, typename Real, int GIJ, bool SCALED, int TERMS, bool SHARE = false>
 static void launch_project_nt(const BaryonProjectArgs &a, const ProjectArgs &pa, size_t tab_bytes,
                               hipStream_t stream)
 {
-  auto kernel = baryon_project_kernel<NT, Real, GIJ, SCALED, TERMS>;
+  auto kernel = baryon_project_kernel<NT, Real, GIJ, SCALED, TERMS, SHARE>;
   unsigned grid = pa.total_items;
   if (project_persistent<Real, NT>()) {
     // as many waves as the chip holds at once: resident blocks per CU (queried
@@ -672,6 +682,16 @@ static void launch_project_g(const BaryonProjectArgs &a, const ProjectArgs &pa, 
     tab = tab > n ? tab : n;
   }
   const size_t tab_bytes = sizeof(double2) * tab;
+  // the shared-P_n form is instantiated for the nucleon's Cg5 diquark at up
+  // to 32 momentum slots; other launches read P_n as a third propagator
+  if constexpr (GIJ == 5)
+    if (pa.share_n && nt <= 2) {
+      if (nt == 1)
+        launch_project_nt<1, Real, GIJ, SCALED, TERMS, true>(a, pa, tab_bytes, stream);
+      else
+        launch_project_nt<2, Real, GIJ, SCALED, TERMS, true>(a, pa, tab_bytes, stream);
+      return;
+    }
   switch (nt) {
   case 1: launch_project_nt<1, Real, GIJ, SCALED, TERMS>(a, pa, tab_bytes, stream); break;
   case 2: launch_project_nt<2, Real, GIJ, SCALED, TERMS>(a, pa, tab_bytes, stream); break;

@@ -62,6 +62,7 @@ struct ProjectArgs {
   int mom[MAX_MOM][3];
   int nreq;
   unsigned total_items; // over all requests
+  int share_n;          // every request has propag_n == propag_i (and exp_n == exp_i)
   ProjectReq req[MAX_BATCH];
 };
 

@@ -444,10 +444,12 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
       for (int mu = 0; mu < 3; ++mu) pa.mom[k][mu] = momenta[3 * slots[k].rep + mu];
     pa.nreq = std::min(nreq - r0, MAX_BATCH);
     size_t items = 0;
+    pa.share_n = 1;
     for (int r = 0; r < pa.nreq; ++r) {
       pa.req[r] = preq[r0 + r];
       pa.req[r].item0 = (unsigned)items;
       items += plan[r0 + r].items;
+      pa.share_n &= pa.req[r].propag_n == pa.req[r].propag_i;
     }
     pa.total_items = (unsigned)items;
     if (precision == CONTRACT_FP64)
