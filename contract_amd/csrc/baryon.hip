@@ -518,17 +518,43 @@ enum { TERMS_RUNTIME = 0, TERMS_NUCLEON = 1 };
 // is read from HBM and staged once per site and serves as both P_i and P_n:
 // 4608 instead of 6912 bytes per site in FP64.  The tails' scratch is then
 // P_j's consumed staging area, since P_i's is still being read as P_n.
+//
+// The FP64 nucleon at one momentum tile runs two waves per SIMD: it stages
+// two propagators at a time (18.5 KB of LDS per wave, eight waves per CU)
+// and, without the register prefetch of the whole next tile, fits half the
+// register file.  Each wave loads P_i (and P_n) at the top of the step and
+// the other wave on the SIMD computes meanwhile; P_n waits in registers
+// while the diquark is formed, then takes P_i's consumed staging area.  The
+// one-wave forms, bound by their own step latency, hide the loads behind the
+// step instead.
+template <typename Real, int NT, int TERMS, bool SHARE> constexpr bool project_two_waves()
+{
+  return sizeof(Real) == 8 && TERMS == TERMS_NUCLEON && NT == 1;
+}
+// the two-wave form prefetches P_j a step ahead (36 VGPRs) and loads P_i at
+// the top of the step
+#ifndef CONTRACT_TWO_WAVE_PFJ
+#define CONTRACT_TWO_WAVE_PFJ 1
+#endif
+
 template <int NT, typename Real, int GIJ = -1, bool SCALED = false, int TERMS = TERMS_RUNTIME, bool SHARE = false>
-__global__ __launch_bounds__(WAVE)
-__attribute__((amdgpu_waves_per_eu(sizeof(Real) == 8 ? CONTRACT_WPE64 : (NT == 1 ? CONTRACT_WPE32 : 1)))) void
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(
+  project_two_waves<Real, NT, TERMS, SHARE>() ? 2
+  : sizeof(Real) == 8                         ? CONTRACT_WPE64
+                                              : (NT == 1 ? CONTRACT_WPE32 : 1)))) void
 baryon_project_kernel(
   BaryonProjectArgs a, ProjectArgs args)
 {
   using C = cplx<Real>;
   constexpr int SL = site_lds<C>();
-  constexpr bool PERSIST = project_persistent<Real, NT>();
-  constexpr int NP = SHARE ? 2 : 3; // distinct propagators staged per site
-  __shared__ C lds[NP][SITES_PER_WAVE * SL];
+  constexpr bool TWO = project_two_waves<Real, NT, TERMS, SHARE>();
+  constexpr bool PREFETCH = !TWO;
+  constexpr bool PERSIST = PREFETCH && project_persistent<Real, NT>();
+  constexpr bool PF_J = TWO && CONTRACT_TWO_WAVE_PFJ; // two-wave form: P_j a step ahead
+  static_assert(!TWO || !SCALED, "the two-wave form runs on FP64 storage");
+  constexpr int NP = SHARE ? 2 : 3; // distinct propagators read per site
+  constexpr int NL = TWO ? 2 : NP;  // staging areas
+  __shared__ C lds[NL][SITES_PER_WAVE * SL];
   extern __shared__ double2 tab[]; // Gx + Gy + Gz phase entries, sized at launch
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
@@ -554,13 +580,15 @@ baryon_project_kernel(
   };
   Tile<C> buf[NP];
   int ex_buf = 0; // exponent sum of the tile in buf
-  {
+  if constexpr (PREFETCH) {
     const ProjectReq &pa = args.req[r];
     const int n = tile_sites(first, last);
     buf[0].load((const C *)pa.propag_i, first, n, lane);
     buf[1].load((const C *)pa.propag_j, first, n, lane);
     if constexpr (!SHARE) buf[NP - 1].load((const C *)pa.propag_n, first, n, lane);
     if constexpr (SCALED) ex_buf = site_exp(pa, first, n);
+  } else if constexpr (PF_J) {
+    buf[1].load((const C *)args.req[r].propag_j, first, tile_sites(first, last), lane);
   }
 
   for (;;) {
@@ -587,13 +615,21 @@ baryon_project_kernel(
       for (int u = 0; u < 4; ++u) acc[tt][u] = dbl4{0.0, 0.0, 0.0, 0.0};
 
     for (size_t s = first; s < last; s += SITES_PER_WAVE) {
+      if constexpr (TWO) {
+        const int n = tile_sites(s, last);
+        buf[0].load(Pi, s, n, lane);
+        if constexpr (!PF_J) buf[1].load(Pj, s, n, lane);
+        if constexpr (!SHARE) buf[2].load(Pn, s, n, lane); // staged after the diquark
+      }
 #pragma unroll
-      for (int p = 0; p < NP; ++p) buf[p].store(lds[p], lane);
+      for (int p = 0; p < NL; ++p) buf[p].store(lds[p], lane);
       __syncthreads();
       int ex = ex_buf;
       const size_t next = s + SITES_PER_WAVE;
       const bool here = next < last; // else the next item's first tile (uniform)
-      if (here || more) {
+      if constexpr (PF_J)
+        if (here) buf[1].load(Pj, next, tile_sites(next, last), lane);
+      if (PREFETCH && (here || more)) {
         const size_t st = here ? next : first_next;
         const int n = here ? tile_sites(next, last) : tile_sites(first_next, last_next);
         buf[0].load(here ? Pi : Qi, st, n, lane);
@@ -602,8 +638,8 @@ baryon_project_kernel(
         if constexpr (SCALED) ex_buf = site_exp(here ? pa : args.req[r_next], st, n);
       }
 
-      C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[SHARE ? 0 : 2] + g * SL;
-      C *scr = SHARE ? sj : si; // consumed once D is formed
+      C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[NL == 2 ? 0 : 2] + g * SL;
+      C *scr = NL == 2 ? sj : si; // consumed once D is formed
       if constexpr (sizeof(Real) == 4 && !SCALED) {
         // P_i and P_j only: D of the rescaled blocks is O(1), so the tails'
         // products with P_n are as large as P_n's own entries and cannot
@@ -618,6 +654,11 @@ baryon_project_kernel(
         diquark_tensor_static<GIJ>(D, si, sj, q);
       else
         diquark_tensor(D, si, sj, q, a.gij);
+      if constexpr (TWO && !SHARE) { // P_n into P_i's consumed area
+        __syncthreads();
+        buf[2].store(lds[0], lane);
+        __syncthreads();
+      }
       double2 R = make_double2(0.0, 0.0);
       if constexpr (TERMS == TERMS_NUCLEON) { // types {IK_JL_NM, IM_JL_NK}, in that order
         const C r0 = baryon_tail<0, C>(D, sn, scr, lane, kl);
@@ -653,7 +694,7 @@ static void launch_project_nt(const BaryonProjectArgs &a, const ProjectArgs &pa,
 {
   auto kernel = baryon_project_kernel<NT, Real, GIJ, SCALED, TERMS, SHARE>;
   unsigned grid = pa.total_items;
-  if (project_persistent<Real, NT>()) {
+  if (project_persistent<Real, NT>() && !project_two_waves<Real, NT, TERMS, SHARE>()) {
     // as many waves as the chip holds at once: resident blocks per CU (queried
     // once per device, with the largest phase table) x CUs
     static std::atomic<int> cache[64];
@@ -703,11 +744,12 @@ static void launch_project_g(const BaryonProjectArgs &a, const ProjectArgs &pa, 
 template <typename Real, bool SCALED = false>
 static void launch_project_t(const BaryonProjectArgs &a, const ProjectArgs &pa, hipStream_t stream)
 {
-  // the FP64 kernel holds one wave per SIMD either way and measured 1% faster
-  // in the runtime form, so only the FP32 forms take the nucleon kernel
-  const bool nucleon = sizeof(Real) == 4 && a.nterm == 2 && a.types[0] == 0 && a.types[1] == 5;
-  if constexpr (sizeof(Real) == 4)
-    if (a.gamma_ij == 5 && nucleon) return launch_project_g<Real, 5, SCALED, TERMS_NUCLEON>(a, pa, stream);
+  // FP64 takes the nucleon kernel where it runs two waves per SIMD (one
+  // momentum tile) or shares P_n; at more tiles the one-wave FP64 kernels
+  // measured 1% faster in the runtime form
+  const bool nucleon = a.nterm == 2 && a.types[0] == 0 && a.types[1] == 5 &&
+                       (sizeof(Real) == 4 || pa.share_n || pa.np <= 16);
+  if (a.gamma_ij == 5 && nucleon) return launch_project_g<Real, 5, SCALED, TERMS_NUCLEON>(a, pa, stream);
   if (a.gamma_ij == 5)
     launch_project_g<Real, 5, SCALED, TERMS_RUNTIME>(a, pa, stream);
   else
