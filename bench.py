@@ -648,7 +648,7 @@ def main():
         cpu = cpu_baseline(torch, (u, d, u2), gpu_sample)
 
     achieved = BYTES_PER_SITE_FP64 * V / (kern64 * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic("baryon_project_kernel<1,double,5>")
+    traffic, traffic_src = measured_traffic("baryon_project_kernel<1,double,5,nucleon>")
     result = {
         "metric": "baryon 2pt sites/s & % HBM roofline, FP32 vs FP64; p50/p99 request latency",
         "value": V / (wall64 * 1e-3),
@@ -679,7 +679,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "baryon_project_kernel<1, double, 5, false, 0>",
+            "kernel": "baryon_project_kernel<1, double, 5, false, 1, false>",
             "kernel_ms": kern64,
             "bytes_per_site": BYTES_PER_SITE_FP64,
         },

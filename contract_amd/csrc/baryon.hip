@@ -519,41 +519,43 @@ enum { TERMS_RUNTIME = 0, TERMS_NUCLEON = 1 };
 // 4608 instead of 6912 bytes per site in FP64.  The tails' scratch is then
 // P_j's consumed staging area, since P_i's is still being read as P_n.
 //
-// The FP64 nucleon at one momentum tile runs two waves per SIMD: it stages
-// two propagators at a time (18.5 KB of LDS per wave, eight waves per CU)
-// and, without the register prefetch of the whole next tile, fits half the
-// register file.  Each wave loads P_i (and P_n) at the top of the step and
-// the other wave on the SIMD computes meanwhile; P_n waits in registers
-// while the diquark is formed, then takes P_i's consumed staging area.  The
-// one-wave forms, bound by their own step latency, hide the loads behind the
-// step instead.
-template <typename Real, int NT, int TERMS, bool SHARE> constexpr bool project_two_waves()
+// The staged form (the FP64 nucleon at one momentum tile) holds two
+// propagators in LDS at a time and skips the register prefetch of the whole
+// next tile, which buys a second wave per SIMD (18.5 KB of LDS per wave,
+// eight waves per CU, half the register file).  Each wave loads P_i (and
+// P_n) at the top of the step while the other wave on the SIMD computes;
+// P_n waits in registers while the diquark is formed, then takes P_i's
+// consumed staging area.  The one-wave FP64 forms, bound by their own step
+// latency, hide the loads behind the step instead.  (The FP32 forms already
+// run two waves per SIMD with the full prefetch; staged at three waves they
+// measured 8-17% slower.)
+template <typename Real, int NT, int TERMS> constexpr bool project_staged()
 {
   return sizeof(Real) == 8 && TERMS == TERMS_NUCLEON && NT == 1;
 }
-// the two-wave form prefetches P_j a step ahead (36 VGPRs) and loads P_i at
-// the top of the step
-#ifndef CONTRACT_TWO_WAVE_PFJ
-#define CONTRACT_TWO_WAVE_PFJ 1
+// the staged form prefetches P_j a step ahead and loads P_i at the top of
+// the step
+#ifndef CONTRACT_STAGED_PFJ
+#define CONTRACT_STAGED_PFJ 1
 #endif
 
 template <int NT, typename Real, int GIJ = -1, bool SCALED = false, int TERMS = TERMS_RUNTIME, bool SHARE = false>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(
-  project_two_waves<Real, NT, TERMS, SHARE>() ? 2
-  : sizeof(Real) == 8                         ? CONTRACT_WPE64
-                                              : (NT == 1 ? CONTRACT_WPE32 : 1)))) void
+  project_staged<Real, NT, TERMS>() ? 2
+  : sizeof(Real) == 8               ? CONTRACT_WPE64
+                                    : (NT == 1 ? CONTRACT_WPE32 : 1)))) void
 baryon_project_kernel(
   BaryonProjectArgs a, ProjectArgs args)
 {
   using C = cplx<Real>;
   constexpr int SL = site_lds<C>();
-  constexpr bool TWO = project_two_waves<Real, NT, TERMS, SHARE>();
-  constexpr bool PREFETCH = !TWO;
+  constexpr bool STAGED = project_staged<Real, NT, TERMS>();
+  constexpr bool PREFETCH = !STAGED;
   constexpr bool PERSIST = PREFETCH && project_persistent<Real, NT>();
-  constexpr bool PF_J = TWO && CONTRACT_TWO_WAVE_PFJ; // two-wave form: P_j a step ahead
-  static_assert(!TWO || !SCALED, "the two-wave form runs on FP64 storage");
-  constexpr int NP = SHARE ? 2 : 3; // distinct propagators read per site
-  constexpr int NL = TWO ? 2 : NP;  // staging areas
+  constexpr bool PF_J = STAGED && CONTRACT_STAGED_PFJ; // staged form: P_j a step ahead
+  constexpr int NP = SHARE ? 2 : 3;   // distinct propagators read per site
+  constexpr int NL = STAGED ? 2 : NP; // staging areas
+  constexpr bool MID_N = STAGED && !SHARE; // P_n staged after the diquark
   __shared__ C lds[NL][SITES_PER_WAVE * SL];
   extern __shared__ double2 tab[]; // Gx + Gy + Gz phase entries, sized at launch
   const int lane = threadIdx.x;
@@ -615,11 +617,12 @@ baryon_project_kernel(
       for (int u = 0; u < 4; ++u) acc[tt][u] = dbl4{0.0, 0.0, 0.0, 0.0};
 
     for (size_t s = first; s < last; s += SITES_PER_WAVE) {
-      if constexpr (TWO) {
+      if constexpr (STAGED) {
         const int n = tile_sites(s, last);
         buf[0].load(Pi, s, n, lane);
         if constexpr (!PF_J) buf[1].load(Pj, s, n, lane);
-        if constexpr (!SHARE) buf[2].load(Pn, s, n, lane); // staged after the diquark
+        if constexpr (!SHARE) buf[2].load(Pn, s, n, lane);
+        if constexpr (SCALED) ex_buf = site_exp(pa, s, n);
       }
 #pragma unroll
       for (int p = 0; p < NL; ++p) buf[p].store(lds[p], lane);
@@ -654,7 +657,7 @@ baryon_project_kernel(
         diquark_tensor_static<GIJ>(D, si, sj, q);
       else
         diquark_tensor(D, si, sj, q, a.gij);
-      if constexpr (TWO && !SHARE) { // P_n into P_i's consumed area
+      if constexpr (MID_N) { // P_n into P_i's consumed area
         __syncthreads();
         buf[2].store(lds[0], lane);
         __syncthreads();
@@ -694,7 +697,7 @@ static void launch_project_nt(const BaryonProjectArgs &a, const ProjectArgs &pa,
 {
   auto kernel = baryon_project_kernel<NT, Real, GIJ, SCALED, TERMS, SHARE>;
   unsigned grid = pa.total_items;
-  if (project_persistent<Real, NT>() && !project_two_waves<Real, NT, TERMS, SHARE>()) {
+  if (project_persistent<Real, NT>() && !project_staged<Real, NT, TERMS>()) {
     // as many waves as the chip holds at once: resident blocks per CU (queried
     // once per device, with the largest phase table) x CUs
     static std::atomic<int> cache[64];
