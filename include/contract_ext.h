@@ -78,7 +78,9 @@ size_t contract_project_workspace_bytes(const int dims[4], int site_order, int n
 
 /* Baryon two-point, summed over `nterm` contraction terms (e.g. the direct and
  * exchange terms of a proton), for `nproj` insertions, projected to
- * timeslices and momenta.  contract_types[t] in 0..5 (BaryonContractType). */
+ * timeslices and momenta.  contract_types[t] in 0..5 (BaryonContractType).
+ * propag_n may be the same buffer as propag_i (a nucleon's (u, d, u)): it is
+ * then read once per site, with results identical to passing a copy. */
 int contract_baryon_twopoint_project(void *out, const void *propag_i, const void *propag_j, const void *propag_n,
                                      int precision, const int *contract_types, const double _Complex *weights,
                                      int nterm, int gamma_ij, int gamma_kl, const double _Complex *projectors,
