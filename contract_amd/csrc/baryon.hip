@@ -286,27 +286,26 @@ __device__ __forceinline__ int nonneg_mod(int a, int L)
 // k_mu = (p_mu x_mu) mod L_mu move by fixed increments (a wrap of x_mu by L_mu
 // leaves k_mu unchanged).  A slice is [z][y][xr] with xr = x (lexicographic)
 // or x / 2 (even-odd: x = 2 xr + ((parity + y + z + t) & 1)).
-// Remainder of 0 <= a < 2^24 by 0 < L, from a float reciprocal: the quotient
-// estimate is off by at most one, which the two selects correct.
-__device__ __forceinline__ int mod_small(int a, int L, float inv_l)
+// (a + b) mod G for 0 <= a, b < G
+__device__ __forceinline__ int add_mod(int a, int b, int G)
 {
-  int r = a - L * (int)((float)a * inv_l);
-  r += r < 0 ? L : 0;
-  r -= r >= L ? L : 0;
-  return r;
+  const int r = a + b;
+  return r >= G ? r - G : r;
 }
 
 template <int NT> struct PhaseWalk {
   static_assert(MAX_PHASE_TAB < 512, "table indices are packed in 9-bit fields");
   int xr, y, z;  // lane coordinates (local)
   int Lr, pbase; // row length of a slice, parity + t (even-odd)
-  float inv_gx, inv_gy;
   // per momentum tile, three 9-bit fields per register (global extents G_mu < 512):
   //   k = kx | ky << 9 | kz << 18, kx = p_x (xs xr + o_x) mod Gx with xs = 2 for
   //   even-odd (the parity bit is added in phase()) and 1 otherwise,
   //   ky = p_y (y + o_y) mod Gy, kz = p_z (z + o_z) mod Gz;
-  //   m = p_x | p_y << 9 | p_z << 18 (p_mu mod G_mu; 0 for rows past np)
-  int k[NT], m[NT];
+  //   m = p_x | p_y << 9 | p_z << 18 (p_mu mod G_mu; 0 for rows past np);
+  //   d = the increments, each reduced mod G_mu: xs * 4 * p_x per 4-site
+  //   step | -p_x Lx when x wraps (y steps on) << 9 | -p_y Ly when y wraps
+  //   (z steps on) << 18, so a step is adds and conditional subtracts
+  int k[NT], m[NT], d[NT];
 
   __device__ __forceinline__ void init(const ProjectArgs &args, const ProjectReq &pa, size_t first, int g, int q)
   {
@@ -315,8 +314,6 @@ template <int NT> struct PhaseWalk {
     const int lt = pa.dims[3];
     pbase = (int)(slice / lt) + (int)(slice % lt);
     Lr = pa.order ? pa.dims[0] >> 1 : pa.dims[0];
-    inv_gx = 1.0f / (float)pa.gdims[0];
-    inv_gy = 1.0f / (float)pa.gdims[1];
     xr = (int)(r0 % (unsigned)Lr);
     const unsigned rest = r0 / (unsigned)Lr;
     y = (int)(rest % (unsigned)pa.dims[1]);
@@ -334,6 +331,11 @@ template <int NT> struct PhaseWalk {
       const int kz = (mz * ((z + pa.origin[2]) % pa.gdims[2])) % pa.gdims[2];
       k[tt] = kx | ky << 9 | kz << 18;
       m[tt] = mx | my << 9 | mz << 18;
+      const int gx = pa.gdims[0], gy = pa.gdims[1];
+      const int dkx = (xs * SITES_PER_WAVE * mx) % gx;
+      const int wx = (gx - (mx * pa.dims[0]) % gx) % gx;
+      const int wy = (gy - (my * pa.dims[1]) % gy) % gy;
+      d[tt] = dkx | wx << 9 | wy << 18;
     }
   }
 
@@ -350,13 +352,9 @@ template <int NT> struct PhaseWalk {
 
   __device__ __forceinline__ void advance(const ProjectReq &pa)
   {
-    const int xs = pa.order ? 2 : 1;
     const int Gx = pa.gdims[0], Gy = pa.gdims[1], Gz = pa.gdims[2];
 #pragma unroll
-    for (int tt = 0; tt < NT; ++tt) {
-      const int kx = mod_small((k[tt] & 511) + xs * SITES_PER_WAVE * (m[tt] & 511), Gx, inv_gx);
-      k[tt] = (k[tt] & ~511) | kx;
-    }
+    for (int tt = 0; tt < NT; ++tt) k[tt] = (k[tt] & ~511) | add_mod(k[tt] & 511, d[tt] & 511, Gx);
     xr += SITES_PER_WAVE;
     while (xr >= Lr) {
       // x steps back by Lx: kx -= p_x Lx (0 mod Gx for a whole lattice), y steps on
@@ -364,21 +362,18 @@ template <int NT> struct PhaseWalk {
       ++y;
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
-        const int mx = m[tt] & 511;
-        const int kx = mod_small((k[tt] & 511) + Gx - mod_small(mx * pa.dims[0], Gx, inv_gx), Gx, inv_gx);
-        int ky = ((k[tt] >> 9) & 511) + ((m[tt] >> 9) & 511);
-        ky -= ky >= Gy ? Gy : 0;
+        const int kx = add_mod(k[tt] & 511, (d[tt] >> 9) & 511, Gx);
+        const int ky = add_mod((k[tt] >> 9) & 511, (m[tt] >> 9) & 511, Gy);
         k[tt] = (k[tt] & ~((1 << 18) - 1)) | ky << 9 | kx;
       }
       if (y == pa.dims[1]) {
+        // y steps back by Ly: ky -= p_y Ly, z steps on
         y = 0;
         ++z;
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) {
-          const int my = (m[tt] >> 9) & 511;
-          const int ky = mod_small(((k[tt] >> 9) & 511) + Gy - mod_small(my * pa.dims[1], Gy, inv_gy), Gy, inv_gy);
-          int kz = (k[tt] >> 18) + (m[tt] >> 18);
-          kz -= kz >= Gz ? Gz : 0;
+          const int ky = add_mod((k[tt] >> 9) & 511, d[tt] >> 18, Gy);
+          const int kz = add_mod(k[tt] >> 18, m[tt] >> 18, Gz);
           k[tt] = (k[tt] & 511) | ky << 9 | kz << 18;
         }
       }
