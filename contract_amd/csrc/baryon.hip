@@ -210,27 +210,55 @@ __device__ __forceinline__ C baryon_tail_rt(int type, const C (&D)[9], const C *
 // baryon_two_point passes W = G(gamma_mn), baryon_general_two_point passes
 // W = project_mn (include/baryon_general.cuh:62-67).
 // ---------------------------------------------------------------------------
+// Staged like the fused nucleon kernel: two propagators in LDS at a time
+// (18.5 KB per wave instead of 27.8 KB), so eight waves fit per CU, two per
+// SIMD.  P_j is prefetched a tile ahead; P_i and P_n are loaded at the top of
+// the tile, P_n waits in registers while the diquark is formed and then takes
+// P_i's consumed area; the other wave on the SIMD covers the loads.
 template <int T, int GIJ>
-__global__ __launch_bounds__(WAVE) void baryon_site_kernel(BaryonSiteArgs a)
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void baryon_site_kernel(
+  BaryonSiteArgs a)
 {
   using C = double2;
   constexpr int SL = site_lds<C>();
-  __shared__ C lds[3 * SITES_PER_WAVE * SL];
+  __shared__ C lds[2][SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
   const C w = cmake<C>(a.w_re[q], a.w_im[q]);
   const KlLane<C> kl(a.gkl, a.gklT, q);
-  const C *P[3] = {(const C *)a.propag_i, (const C *)a.propag_j, (const C *)a.propag_n};
-  persistent_tiles<3>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
+  const C *Pi = (const C *)a.propag_i, *Pj = (const C *)a.propag_j, *Pn = (const C *)a.propag_n;
+  const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
+  auto nsites_of = [&](size_t t) {
+    const size_t left = a.volume - t * SITES_PER_WAVE;
+    return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
+  };
+  C *si = lds[0] + g * SL, *sj = lds[1] + g * SL;
+  Tile<C> bi, bj, bn;
+  size_t tile = blockIdx.x;
+  if (tile < ntiles) bj.load(Pj, tile * SITES_PER_WAVE, nsites_of(tile), lane);
+  for (; tile < ntiles; tile += gridDim.x) {
+    const size_t first = tile * SITES_PER_WAVE;
+    const int nsites = nsites_of(tile);
+    bi.load(Pi, first, nsites, lane);
+    bn.load(Pn, first, nsites, lane);
+    bi.store(lds[0], lane);
+    bj.store(lds[1], lane);
+    __syncthreads();
+    const size_t next = tile + gridDim.x;
+    if (next < ntiles) bj.load(Pj, next * SITES_PER_WAVE, nsites_of(next), lane);
     C D[9];
     if constexpr (GIJ >= 0)
-      diquark_tensor_static<GIJ>(D, lds + g * SL, lds + (SITES_PER_WAVE + g) * SL, q);
+      diquark_tensor_static<GIJ>(D, si, sj, q);
     else
-      diquark_tensor(D, lds + g * SL, lds + (SITES_PER_WAVE + g) * SL, q, a.gij);
-    const C R = baryon_tail<T, C, false>(D, lds + (2 * SITES_PER_WAVE + g) * SL, lds + g * SL, lane, kl);
+      diquark_tensor(D, si, sj, q, a.gij);
+    __syncthreads();
+    bn.store(lds[0], lane);
+    __syncthreads();
+    const C R = baryon_tail<T, C, false>(D, si, sj, lane, kl);
     const C c = group16_allreduce(cmul(w, R));
     if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
-  });
+    __syncthreads();
+  }
 }
 
 template <int GIJ>
@@ -251,7 +279,7 @@ void launch_baryon_site(int type, const BaryonSiteArgs &a, hipStream_t stream)
 {
   const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
   if (ntiles == 0) return;
-  const dim3 grid((unsigned)persistent_grid(ntiles, 5));
+  const dim3 grid((unsigned)persistent_grid(ntiles, 8));
   if (a.gamma_ij == 5)
     launch_site_g<5>(type, a, grid, stream);
   else
