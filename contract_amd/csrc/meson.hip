@@ -47,8 +47,8 @@ void launch_meson(const MesonArgs &a, hipStream_t stream)
 // Gij = G(g)) for all g at once (include/meson_all.cuh:66-72,96-102).  Lane
 // (i, l) forms the four colour traces that any of the sixteen running gammas
 // can select, weights them, and a 16-lane reduce-scatter leaves correlator g
-// in lane g.
-__global__ __launch_bounds__(WAVE) void meson_all_kernel(MesonAllArgs a)
+// in lane g.  SINK is compile-time: one form's registers per kernel.
+template <bool SINK> __global__ __launch_bounds__(WAVE) void meson_all_kernel(MesonAllArgs a)
 {
   using C = double2;
   constexpr int SL = site_lds<C>();
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(WAVE) void meson_all_kernel(MesonAllArgs a)
     const C *si = lds + g * SL, *sj = lds + (SITES_PER_WAVE + g) * SL;
     const int i = q >> 2, l = q & 3;
     C x[9], y[9], K[4], v[16];
-    if (!a.sink) {
+    if constexpr (!SINK) {
       // fixed row pair (i, j = a(i)); running k
       const int j = sparse_col(a.fixed, i);
       load_block(y, sj, j, l);
@@ -107,7 +107,11 @@ void launch_meson_all(const MesonAllArgs &a, hipStream_t stream)
 {
   const size_t blocks = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
   if (blocks == 0) return;
-  hipLaunchKernelGGL(meson_all_kernel, dim3((unsigned)persistent_grid(blocks, 8)), dim3(WAVE), 0, stream, a);
+  const dim3 grid((unsigned)persistent_grid(blocks, 8));
+  if (a.sink)
+    hipLaunchKernelGGL(meson_all_kernel<true>, grid, dim3(WAVE), 0, stream, a);
+  else
+    hipLaunchKernelGGL(meson_all_kernel<false>, grid, dim3(WAVE), 0, stream, a);
 }
 
 } // namespace contract_amd
