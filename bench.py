@@ -362,11 +362,13 @@ def precision_study(torch, pc, dev, steps, warmup, hip, timer):
         "fp32_sites_per_s": V / (wall32 * 1e-3),
         "fp64_sites_per_s": V / (wall64 * 1e-3),
         "fp32_speedup": wall64 / wall32,
+        "fp32_kernel_speedup": kern64 / kern32 if kern32 and kern64 else None,
         "fp32_achieved_GBs": BYTES_PER_SITE_FP32 * V / (kern32 * 1e-3) / 1e9,
         "fp64_achieved_GBs": BYTES_PER_SITE_FP64 * V / (kern64 * 1e-3) / 1e9,
         "fp32_scaled": {"max_per_timeslice_rel_err": float(per_t_s.max()),
                         "median_per_timeslice_rel_err": float(np.median(per_t_s)),
-                        "kernel_ms": kern32s, "sites_per_s": V / (wall32s * 1e-3), "speedup": wall64 / wall32s},
+                        "kernel_ms": kern32s, "sites_per_s": V / (wall32s * 1e-3), "speedup": wall64 / wall32s,
+                        "kernel_speedup": kern64 / kern32s if kern32s and kern64 else None},
         "oracle_s": t_oracle,
     }
 
