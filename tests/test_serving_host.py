@@ -113,3 +113,33 @@ def test_catalogue_seeds_are_stable():
     k = PropKey((16, 16, 16, 32), 2, t0=14)
     assert k.seed == PropKey((16, 16, 16, 32), 2, t0=14).seed
     assert k.seed != PropKey((16, 16, 16, 32), 3, t0=14).seed
+
+
+def test_meson_momentum_groups_pair_plus_minus_k():
+    """The serving loop cuts the union of meson momenta into grid passes of
+    at most 8 +-k slots: every momentum lands in exactly one pass, a momentum
+    and its negative share a pass, and the 27 momenta of {-1,0,1}^3 (13 pairs
+    and the origin) take two passes instead of four groups of 8."""
+    from contract_amd.serving import MOM27, Engine
+    groups = Engine._momentum_groups(sorted(MOM27))
+    assert [len(g) for g in groups] == [16, 11]
+    flat = [m for g in groups for m in g]
+    assert sorted(flat) == sorted(MOM27) and len(set(flat)) == 27
+    for g in groups:
+        slots = {min(m, tuple(-v for v in m)) for m in g}
+        assert len(slots) <= 8
+        for m in g:
+            neg = tuple(-v for v in m)
+            assert neg in g  # every momentum of {-1,0,1}^3 has its negative in the union
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        pick = sorted({MOM27[i] for i in rng.choice(27, size=int(rng.integers(1, 28)), replace=False)})
+        groups = Engine._momentum_groups(pick)
+        flat = [m for g in groups for m in g]
+        assert sorted(flat) == pick
+        assert all(len({min(m, tuple(-v for v in m)) for m in g}) <= 8 for g in groups)
+        # pairs present in the union are never split across passes
+        for g in groups:
+            for m in g:
+                neg = tuple(-v for v in m)
+                assert neg not in pick or neg in g
