@@ -117,3 +117,15 @@ def test_batches_with_and_without_sharing(dev):
     torch.cuda.synchronize()
     for s, a, b in zip(singles, got_shared, got_mixed):
         assert torch.equal(s, a) and torch.equal(s, b)
+
+
+@pytest.mark.parametrize("kind,tol", [("fp64", 1e-12), ("scaled", 1e-6)])
+@pytest.mark.parametrize("order", [0, 1])
+def test_ragged_slices_shared(dev, kind, tol, order):
+    """27 sites per timeslice (lexicographic 3^3x5) or per parity slice
+    (even-odd 6x3x3x4): work items end part-way through a 4-site tile in the
+    staged FP64 kernel and the FP32 forms, with P_n = P_i."""
+    dims = (3, 3, 3, 5) if order == 0 else (6, 3, 3, 4)
+    shared, copy, want = shared_and_copy(dev, dims, PROTON, [0, 8], MOM7, order, kind)
+    assert np.array_equal(shared, copy)
+    assert per_timeslice_error(shared, want) < tol
