@@ -6,7 +6,9 @@ The reference is HIP-only, so this runs on a GPU machine from the repo root,
 after `make -C oracle ref` built oracle/_ref/libcontract_ref.so from the
 reference sources:
 
-    python tests/golden/make_ref_vectors.py gpurun_out/ref_vectors.npz
+    python tests/golden/make_ref_vectors.py OUT/ref_vectors.npz
+
+and the file is then copied to tests/golden/.
 
 Inputs are golden.npz's pi, pj, pn (2^3x4 lattice, V = 32).  Only data is
 written: the inputs' identity (by name) and the reference's outputs.
