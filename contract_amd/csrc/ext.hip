@@ -78,8 +78,11 @@ std::vector<MomSlot> pair_momenta(const int *mom, int nmom)
 }
 
 constexpr size_t MIN_ITEM_SITES = 128;
+// 8 work items per CU: one item per resident wave of the two-wave staged
+// kernel (one round, no tail of half-empty rounds); 16 measured 3% slower
+// for the FP64 proton at 24^3x64 and 0-1.5% slower for the other forms
 #ifndef CONTRACT_ITEMS_PER_CU
-#define CONTRACT_ITEMS_PER_CU 16
+#define CONTRACT_ITEMS_PER_CU 8
 #endif
 constexpr size_t ITEMS_PER_CU = CONTRACT_ITEMS_PER_CU;
 

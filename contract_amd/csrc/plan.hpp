@@ -36,7 +36,7 @@ struct MomSlot {
 };
 std::vector<MomSlot> pair_momenta(const int *mom, int nmom);
 
-// sites_per_item <= 0 picks ~16 items per compute unit; nslot momentum slots.
+// sites_per_item <= 0 picks ~8 items per compute unit; nslot momentum slots.
 ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item);
 int check_lattice(const int dims[4], int order);
 int check_momenta(const int *momenta, int nmom);
