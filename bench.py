@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import gc
 import json
 import os
 import sys
@@ -106,20 +107,30 @@ def measured_traffic(kernel):
 
 
 def time_steps(torch, fn, steps, warmup, hip=None, timer=None):
-    """Run warmup + timed steps; returns (wall ms/step, kernel ms/step or None)."""
+    """Run warmup + timed steps; returns (wall ms/step, kernel ms/step or None).
+    Python's garbage collector is paused over the timed steps (as timeit
+    does): a collection while the host enqueues sub-millisecond steps left
+    the GPU idle and doubled a step's wall time in some runs."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     ev = []
     if hip is not None:
         ev = [(hip.create(), hip.create()) for _ in range(steps)]
-    t0 = time.perf_counter()
-    for k in range(steps):
-        if ev:
-            timer(ev[k][0], ev[k][1])
-        fn()
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) * 1e3 / steps
+    gc_on = gc.isenabled()
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for k in range(steps):
+            if ev:
+                timer(ev[k][0], ev[k][1])
+            fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+    finally:
+        if gc_on:
+            gc.enable()
     kern = None
     if ev:
         timer(None, None)
