@@ -130,6 +130,11 @@ size_t max_group_items(const contract_meson_request *reqs, int nreq, int chunk)
 
 using namespace contract_amd;
 
+// Row tiles of each work item's partial: two when a call may have more than
+// eight momentum slots (its passes then take up to sixteen), from the
+// momentum count alone so the workspace size needs no momenta.
+static int partial_tiles(int nmom) { return nmom > GRID_MOM_PER_TILE ? GRID_MAX_TILES : 1; }
+
 extern "C" {
 
 size_t contract_meson_grid_workspace_bytes(const contract_meson_request *reqs, int nreq, int nmom,
@@ -139,7 +144,7 @@ size_t contract_meson_grid_workspace_bytes(const contract_meson_request *reqs, i
   if (check_tuning(g, CONTRACT_FP64) != CONTRACT_OK || check_requests(reqs, nreq) != CONTRACT_OK || nmom < 1 ||
       nmom > MAX_MOM)
     return 0;
-  return max_group_items(reqs, nreq, chunk_of(g)) * GRID_PARTIAL * sizeof(double);
+  return max_group_items(reqs, nreq, chunk_of(g)) * partial_tiles(nmom) * GRID_PARTIAL_TILE * sizeof(double);
 }
 
 int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, int precision, const int *momenta,
@@ -162,7 +167,8 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
     set_last_error("batch of %zu work items is too large; raise sites_per_item", group_items);
     return CONTRACT_ERR_ARG;
   }
-  const size_t need = group_items * GRID_PARTIAL * sizeof(double);
+  const int tiles = partial_tiles(nmom);
+  const size_t need = group_items * tiles * GRID_PARTIAL_TILE * sizeof(double);
   void *ws = workspace;
   bool own = false;
   if (ws) {
@@ -190,6 +196,8 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
     a.nreq = nr;
     a.chunk = chunk;
     a.partial = static_cast<double *>(ws);
+    a.partial_tiles = tiles;
+    f.partial_tiles = tiles;
     f.nreq = nr;
     size_t items = 0;
     int blocks = 0, tab = 0;

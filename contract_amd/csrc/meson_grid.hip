@@ -66,9 +66,12 @@ __device__ __forceinline__ int scale_block(float2 (&b)[9])
 //    through LDS once more to become the projection's B operand (4 sites x 16
 //    columns).  Needs FP64 or site-scaled FP32 storage: plain FP32 products of
 //    two steeply decaying propagators could underflow without a per-row scale.
-template <typename Real, int NS, bool SCALED = false, bool KMFMA = false>
+// RT row tiles of 8 momentum slots each (one pass holds up to 8 RT slots):
+// every staged site is read and traced once for all of them.  Two tiles
+// double the projection accumulators, which fit only at one wave per SIMD.
+template <typename Real, int NS, bool SCALED = false, bool KMFMA = false, int RT = 1>
 __global__ __launch_bounds__(GRID_THREADS)
-__attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 : 2))) void meson_grid_kernel(
+__attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !KMFMA)) ? 1 : 2))) void meson_grid_kernel(
     MesonGridArgs a)
 {
   static_assert(NS % SITES_PER_WAVE == 0, "tile is a whole number of MFMA site steps");
@@ -136,11 +139,16 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
       sincospi(2.0 * (double)((m * ((k + R.origin[mu]) % G)) % G) / (double)G, &sn, &cs);
       tab[e] = make_double2(cs, -sn);
     }
-    // this lane's MFMA row: momentum q >> 1, phase part q & 1 (0 re, 1 im)
-    const int pr = q >> 1;
-    const bool live = pr < a.np;
-    const int prow = live ? pr : 0;
-    const double2 *tx = tab + prow * (Lx + 1), *ty = tab + offy + prow * (Ly + 1), *tz = tab + offz + prow * (Lz + 1);
+    // this lane's MFMA row of tile rt: momentum slot 8 rt + (q >> 1), phase
+    // part q & 1 (0 re, 1 im)
+    bool live[RT];
+    int prow[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      const int pr = rt * GRID_MOM_PER_TILE + (q >> 1);
+      live[rt] = pr < a.np;
+      prow[rt] = live[rt] ? pr : 0;
+    }
 
     // coordinates of the step's first site, advanced incrementally (uniform):
     // a slice is [z][y][xr] with xr = x (lexicographic) or x / 2 (even-odd,
@@ -160,9 +168,11 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
 
     // column tiles of k = w: l = 0..3 (vector traces, lane q = (i, j)) or
     // i = 0..3 (matrix-core traces, lane q = (j, l)); B = K re / K im
-    dbl4 acc_re[4], acc_im[4];
+    dbl4 acc_re[RT][4], acc_im[RT][4];
 #pragma unroll
-    for (int l = 0; l < 4; ++l) acc_re[l] = acc_im[l] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int l = 0; l < 4; ++l) acc_re[rt][l] = acc_im[rt][l] = dbl4{0.0, 0.0, 0.0, 0.0};
 
     // the workgroup stages NS sites of both propagators per step; the next
     // step's chunks are in flight in registers while this one is computed
@@ -209,10 +219,15 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
         }
         z = z < Lz ? z : 0;
         const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
-        double av = 0.0;
-        if (live) {
-          const double2 ph = cmul(cmul(tx[x], ty[y]), tz[z]);
-          av = (q & 1) ? ph.y : ph.x;
+        double av[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          av[rt] = 0.0;
+          if (live[rt]) {
+            const double2 ph = cmul(cmul(tab[prow[rt] * (Lx + 1) + x], tab[offy + prow[rt] * (Ly + 1) + y]),
+                                    tab[offz + prow[rt] * (Lz + 1) + z]);
+            av[rt] = (q & 1) ? ph.y : ph.x;
+          }
         }
         if constexpr (KMFMA) {
           using KAcc = typename std::conditional<sizeof(Real) == 8, dbl4, flt4>::type;
@@ -270,8 +285,11 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
 #pragma unroll
           for (int i2 = 0; i2 < 4; ++i2) {
             const double2 kv = kbuf[kslot][g][(4 * i2 + w) * KROW + q];
-            acc_re[i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, kv.x, acc_re[i2], 0, 0, 0);
-            acc_im[i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, kv.y, acc_im[i2], 0, 0, 0);
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+              acc_re[rt][i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kv.x, acc_re[rt][i2], 0, 0, 0);
+              acc_im[rt][i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kv.y, acc_im[rt][i2], 0, 0, 0);
+            }
           }
           kslot ^= 1;
           continue;
@@ -297,8 +315,11 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
             kr = ldexp(kr, ex);
             ki = ldexp(ki, ex);
           }
-          acc_re[l] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, kr, acc_re[l], 0, 0, 0);
-          acc_im[l] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, ki, acc_im[l], 0, 0, 0);
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            acc_re[rt][l] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kr, acc_re[rt][l], 0, 0, 0);
+            acc_im[rt][l] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], ki, acc_im[rt][l], 0, 0, 0);
+          }
         }
       }
       x0 += NS;
@@ -312,28 +333,39 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 8 && NS == 8 && !KMFMA) ? 1 
     }
     // v_mfma_f64_16x16x4f64: lane (g, q) register v holds D[g + 4v][q];
     // partial column (k l) * 16 + (i j) with k = w
-    double *dst = a.partial + item * (size_t)GRID_PARTIAL;
+    double *dst = a.partial + item * (size_t)a.partial_tiles * GRID_PARTIAL_TILE;
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = g + 4 * v;
-        const int col = KMFMA ? (w * 4 + (q & 3)) * 16 + t * 4 + (q >> 2) : (w * 4 + t) * 16 + q;
-        dst[(row * 2 + 0) * GRID_COLS + col] = acc_re[t][v];
-        dst[(row * 2 + 1) * GRID_COLS + col] = acc_im[t][v];
-      }
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = rt * 16 + g + 4 * v;
+          const int col = KMFMA ? (w * 4 + (q & 3)) * 16 + t * 4 + (q >> 2) : (w * 4 + t) * 16 + q;
+          dst[(row * 2 + 0) * GRID_COLS + col] = acc_re[rt][t][v];
+          dst[(row * 2 + 1) * GRID_COLS + col] = acc_im[rt][t][v];
+        }
   }
+}
+
+template <typename Real, bool SCALED, bool KMFMA, int RT>
+static void launch_grid_rt(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
+{
+  if (tile_sites == 8)
+    hipLaunchKernelGGL((meson_grid_kernel<Real, 8, SCALED, KMFMA, RT>), dim3((unsigned)blocks), dim3(GRID_THREADS),
+                       shmem, stream, a);
+  else
+    hipLaunchKernelGGL((meson_grid_kernel<Real, 4, SCALED, KMFMA, RT>), dim3((unsigned)blocks), dim3(GRID_THREADS),
+                       shmem, stream, a);
 }
 
 template <typename Real, bool SCALED, bool KMFMA>
 static void launch_grid_t(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
 {
-  if (tile_sites == 8)
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 8, SCALED, KMFMA>), dim3((unsigned)blocks), dim3(GRID_THREADS),
-                       shmem, stream, a);
+  if (a.np > GRID_MOM_PER_TILE)
+    launch_grid_rt<Real, SCALED, KMFMA, 2>(a, tile_sites, blocks, shmem, stream);
   else
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 4, SCALED, KMFMA>), dim3((unsigned)blocks), dim3(GRID_THREADS),
-                       shmem, stream, a);
+    launch_grid_rt<Real, SCALED, KMFMA, 1>(a, tile_sites, blocks, shmem, stream);
 }
 
 void launch_meson_grid(const MesonGridArgs &a, int storage, bool mfma_traces, int tile_sites, size_t blocks,
@@ -386,15 +418,15 @@ __global__ __launch_bounds__(FINISH_GROUPS * GRID_COLS) void meson_grid_finish_k
   double s[4] = {0, 0, 0, 0}, cc[4] = {0, 0, 0, 0};
   for (int par = 0; par < nslices; ++par) {
     const size_t slice = (size_t)par * R.lt + t;
-    const double *base = f.partial + (R.item0 + slice * R.chunks_per_slice) * (size_t)GRID_PARTIAL +
-                         (size_t)(4 * p) * GRID_COLS + col;
+    const size_t stride = (size_t)f.partial_tiles * GRID_PARTIAL_TILE;
+    const double *base = f.partial + (R.item0 + slice * R.chunks_per_slice) * stride + (size_t)(4 * p) * GRID_COLS + col;
     // FINISH_UNROLL chunks' loads in flight at once, summed in chunk order
     for (int ch0 = cg; ch0 < R.chunks_per_slice; ch0 += FINISH_GROUPS * FINISH_UNROLL) {
       double v[FINISH_UNROLL][4];
 #pragma unroll
       for (int k = 0; k < FINISH_UNROLL; ++k) {
         const int ch = ch0 + k * FINISH_GROUPS;
-        const double *src = base + (size_t)(ch < R.chunks_per_slice ? ch : ch0) * GRID_PARTIAL;
+        const double *src = base + (size_t)(ch < R.chunks_per_slice ? ch : ch0) * stride;
         // row 2p: cos weight, row 2p + 1: -sin weight; each with K re / K im
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[k][u] = src[u * GRID_COLS];

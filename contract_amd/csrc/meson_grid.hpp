@@ -12,9 +12,14 @@ namespace contract_amd
 
 constexpr int GRID_THREADS = 256;       // four waves share one staged tile
 constexpr int GRID_MAX_REQ = 32;        // requests per launch
-constexpr int GRID_MOM_PER_PASS = 8;    // 16 MFMA rows = 8 momenta x (re, im)
+constexpr int GRID_MOM_PER_TILE = 8;    // 16 MFMA rows = 8 momentum slots x (cos, -sin)
+constexpr int GRID_MAX_TILES = 2;       // row tiles per pass
+constexpr int GRID_MOM_PER_PASS = GRID_MOM_PER_TILE * GRID_MAX_TILES;
 constexpr int GRID_COLS = 256;          // K[i,k,j,l] columns, (k*4 + l)*16 + i*4 + j
-constexpr int GRID_PARTIAL = 16 * 2 * GRID_COLS; // doubles per work item: [row][K re/im][col]
+// doubles per work item and row tile: [row][K re/im][col]; a work item's
+// partial holds `partial_tiles` of them, rows 2p (cos) and 2p + 1 (-sin) of
+// momentum slot p running on across the tiles
+constexpr int GRID_PARTIAL_TILE = 16 * 2 * GRID_COLS;
 
 struct MesonGridReq {
   const void *propag_i, *propag_j;
@@ -30,11 +35,12 @@ struct MesonGridReq {
 struct MesonGridArgs {
   MesonGridReq req[GRID_MAX_REQ];
   int nreq;
-  int np;    // momenta in this pass (<= GRID_MOM_PER_PASS)
+  int np;    // momentum slots in this pass (<= GRID_MOM_PER_PASS)
   int chunk; // sites per work item, a multiple of the tile
   int mom[GRID_MOM_PER_PASS][3];
   unsigned total_items;
-  double *partial; // [total_items][GRID_PARTIAL]
+  int partial_tiles; // row tiles per work item in `partial`, >= the pass's row tiles
+  double *partial;   // [total_items][partial_tiles * GRID_PARTIAL_TILE]
 };
 
 struct MesonGridOut {
@@ -47,6 +53,7 @@ struct MesonGridOut {
 struct MesonGridFinish {
   MesonGridOut req[GRID_MAX_REQ];
   int nreq, np, np_total;
+  int partial_tiles; // as in MesonGridArgs
   // slot p of the pass holds momentum k: output index of +k and of -k (the
   // rows give both, P(+-k) = A +- iB), -1 where that sign was not requested
   int out_plus[GRID_MOM_PER_PASS], out_minus[GRID_MOM_PER_PASS];

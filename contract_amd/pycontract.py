@@ -479,8 +479,8 @@ def mesonGridProject(requests, momenta, tuning=None, outs=None, workspace=None, 
     (complex128 on the device) with out[g1][g2][p][t] = sum_{x in t}
     exp(-i p.x) meson_two_point(propag_i, propag_j; g1, g2)(x).
 
-    All requests go through one persistent launch (per 32 requests and 8
-    momenta).  complex64 propagators (or ScaledPropagator, see toScaledFP32)
+    All requests go through one persistent launch (per 32 requests and 16
+    +-k momentum slots, a momentum and its negative sharing a slot).  complex64 propagators (or ScaledPropagator, see toScaledFP32)
     select the FP32-contraction / FP64-accumulation form; all requests must
     share the storage form."""
     import torch

@@ -292,10 +292,11 @@ class Engine:
     # -- launches --------------------------------------------------------------
     @staticmethod
     def _momentum_groups(union: Sequence[Tuple[int, int, int]]) -> List[List[Tuple[int, int, int]]]:
-        """The union of momenta cut into groups of at most 8 slots, a slot being
-        a momentum together with its negative when both are wanted (the grid
-        kernel accumulates k and -k as one cos/sin pair), so that one grid pass
-        (8 slots) reads every propagator once for up to 16 momenta."""
+        """The union of momenta cut into groups of at most 16 slots, a slot
+        being a momentum together with its negative when both are wanted (the
+        grid kernel accumulates k and -k as one cos/sin pair), so that one grid
+        pass (16 slots) reads every propagator once for up to 32 momenta: the
+        27 momenta of {-1,0,1}^3 take one pass."""
         have = set(union)
         slots, seen = [], set()
         for m in union:
@@ -305,10 +306,10 @@ class Engine:
             slot = [m] if neg == m or neg not in have else [m, neg]
             seen.update(slot)
             slots.append(slot)
-        return [[m for s in slots[g:g + 8] for m in s] for g in range(0, len(slots), 8)]
+        return [[m for s in slots[g:g + 16] for m in s] for g in range(0, len(slots), 16)]
 
     def _launch_mesons(self, reqs: List[Request], stream) -> list:
-        """One batched grid launch per group of 8 momentum slots of the union
+        """One batched grid launch per group of 16 momentum slots of the union
         of the requests' momenta, each over the requests that need one of
         them; then every request gathers its (gamma pair, momentum) entries."""
         torch = self.torch

@@ -136,7 +136,8 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
  *
  * out being device complex128 [16][16][nmom][Lt] of that request.  All
  * requests of one call share precision, momenta and the workspace, and go
- * through one persistent launch per group of 32 requests and 8 momenta, so
+ * through one persistent launch per group of 32 requests and 16 +-k momentum
+ * slots (a momentum and its negative share a slot), so
  * small and large lattices balance across the chip. */
 typedef struct {
   const void *propag_i; /* device [V][4][4][3][3], complex128 or complex64 */

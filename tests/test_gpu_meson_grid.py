@@ -250,3 +250,42 @@ def test_matrix_core_traces_need_fp64_or_scaled(dev):
     reqs, _ = uneven_batch(dev, np.complex64)
     with pytest.raises(RuntimeError, match="trace_unit"):
         pc.mesonGridProject(reqs, MOM7, tuning=(16, 4, 2, 2))
+
+
+@pytest.mark.parametrize("unit", [1, 2])
+@pytest.mark.parametrize("tile", [4, 8])
+def test_two_row_tiles_per_pass(dev, unit, tile):
+    """More than 8 +-k momentum slots in a pass take two MFMA row tiles (27
+    momenta = 14 slots: one pass); an uneven batch in both orders, FP64 and
+    site-scaled FP32, against the oracle, and each request equal alone and in
+    the batch."""
+    torch, pc, cuda = dev
+    reqs, hosts = uneven_batch(dev)
+    outs = [o.cpu().numpy() for o in pc.mesonGridProject(reqs, MOM27, tuning=(16, tile, 2, unit))]
+    for out, h, r in zip(outs, hosts, reqs):
+        assert grid_error(out, oracle_grid(h, MOM27)) < 1e-12
+    alone = pc.mesonGridProject([reqs[3]], MOM27, tuning=(16, tile, 2, unit))[0].cpu().numpy()
+    assert np.array_equal(alone, outs[3])
+    f32, hosts32 = uneven_batch(dev, np.complex64)
+    outs = pc.mesonGridProject(scaled_batch(dev, f32), MOM27, tuning=(16, tile, 2, unit))
+    for out, h in zip(outs, hosts32):
+        assert timeslice_error(out.cpu().numpy(), oracle_grid(h, MOM27)) < 1e-6
+
+
+def test_many_slots_several_two_tile_passes(dev):
+    """64 momenta (55 +-k slots): passes of 16, 16, 16 and 7 slots, the last
+    with one row tile, into one workspace laid out for two tiles per item."""
+    torch, pc, cuda = dev
+    rng = np.random.default_rng(64)
+    moms = [(0, 0, 0)] + [tuple(int(v) for v in rng.integers(-3, 4, 3)) for _ in range(63)]
+    assert len({min(m, tuple(-v for v in m)) for m in moms}) == 55
+    req, host = make_request(dev, (4, 2, 4, 4), 31, 1, 1)
+    tuning = (16, 4, 2, 2)
+    n7 = pc.meson_grid_workspace_bytes([req], 7, tuning)
+    n64 = pc.meson_grid_workspace_bytes([req], 64, tuning)
+    assert n64 == 2 * n7  # two row tiles per work item once a call can exceed 8 slots
+    ws = torch.empty(n64, dtype=torch.uint8, device=cuda)
+    got = pc.mesonGridProject([req], moms, tuning=tuning, workspace=ws)[0].cpu().numpy()
+    assert grid_error(got, oracle_grid(host, moms)) < 1e-12
+    with pytest.raises(RuntimeError, match="workspace"):
+        pc.mesonGridProject([req], moms, tuning=tuning, workspace=ws[:n7])

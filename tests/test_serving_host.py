@@ -117,17 +117,19 @@ def test_catalogue_seeds_are_stable():
 
 def test_meson_momentum_groups_pair_plus_minus_k():
     """The serving loop cuts the union of meson momenta into grid passes of
-    at most 8 +-k slots: every momentum lands in exactly one pass, a momentum
+    at most 16 +-k slots: every momentum lands in exactly one pass, a momentum
     and its negative share a pass, and the 27 momenta of {-1,0,1}^3 (13 pairs
-    and the origin) take two passes instead of four groups of 8."""
+    and the origin) take one pass."""
     from contract_amd.serving import MOM27, Engine
     groups = Engine._momentum_groups(sorted(MOM27))
-    assert [len(g) for g in groups] == [16, 11]
+    assert [len(g) for g in groups] == [27]
+    wide = [(x, y, z) for x in range(-2, 3) for y in range(-2, 3) for z in range(-2, 3)]  # 62 pairs + origin
+    assert [len({min(m, tuple(-v for v in m)) for m in g}) for g in Engine._momentum_groups(wide)] == [16, 16, 16, 15]
     flat = [m for g in groups for m in g]
     assert sorted(flat) == sorted(MOM27) and len(set(flat)) == 27
     for g in groups:
         slots = {min(m, tuple(-v for v in m)) for m in g}
-        assert len(slots) <= 8
+        assert len(slots) <= 16
         for m in g:
             neg = tuple(-v for v in m)
             assert neg in g  # every momentum of {-1,0,1}^3 has its negative in the union
@@ -137,7 +139,7 @@ def test_meson_momentum_groups_pair_plus_minus_k():
         groups = Engine._momentum_groups(pick)
         flat = [m for g in groups for m in g]
         assert sorted(flat) == pick
-        assert all(len({min(m, tuple(-v for v in m)) for m in g}) <= 8 for g in groups)
+        assert all(len({min(m, tuple(-v for v in m)) for m in g}) <= 16 for g in groups)
         # pairs present in the union are never split across passes
         for g in groups:
             for m in g:
