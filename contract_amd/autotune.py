@@ -29,11 +29,14 @@ _cache: Dict[tuple, Tuple[Tuple[int, int, int], float]] = {}
 
 
 def signature(requests, nmom: int) -> tuple:
-    """What the best tuning depends on: dtype, momentum passes and the lattices
-    in batch order (the item space is laid out request by request)."""
+    """What the best tuning depends on: dtype, the kernel form (one MFMA row
+    tile per pass up to 8 momenta, two beyond: a momentum and its negative
+    share a slot, 8 slots a tile), the number of passes (at most 32 momenta
+    each) and the lattices in batch order (the item space is laid out request
+    by request)."""
     dt = str(requests[0][0].dtype)
     lat = tuple((tuple(int(v) for v in r[2]), int(r[3]) if len(r) > 3 else 0) for r in requests)
-    return (dt, (nmom + 7) // 8, lat)
+    return (dt, 1 if nmom <= 8 else 2, (nmom + 31) // 32, lat)
 
 
 def candidates(sites: Sequence[int] = DEFAULT_SITES, tiles: Sequence[int] = DEFAULT_TILES,
