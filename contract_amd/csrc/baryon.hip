@@ -516,10 +516,11 @@ __device__ __forceinline__ int normalise_site(float2 *site_lds, int q)
 #define CONTRACT_WPE32 2
 #endif
 
-// FP64 with up to two momentum tiles (one wave per SIMD) runs persistent
-// waves that prefetch across work items.  FP32 (two waves per SIMD) and the
-// larger FP64 tiles have no register room for the cross-item state: they run
-// one wave per item.
+// The one-wave FP64 forms with up to two momentum tiles (the runtime term
+// dispatch; the nucleon at two tiles) run persistent waves that prefetch
+// across work items.  The staged FP64 nucleon (one tile, two waves per SIMD,
+// below), FP32 (two waves per SIMD) and the larger FP64 tiles have no
+// register room for the cross-item state: they run one wave per item.
 #ifndef CONTRACT_PROJECT_PERSIST
 #define CONTRACT_PROJECT_PERSIST 1
 #endif
