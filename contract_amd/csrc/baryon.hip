@@ -533,8 +533,9 @@ template <typename Real, int NT> constexpr bool project_persistent()
 // one of the six tails; with all six inlined into the step the register
 // allocator must hold the union of their live ranges.  The nucleon's direct +
 // exchange pair (IK_JL_NM, IM_JL_NK, the proton/neutron two-point of
-// baryon_2pt.py) is instantiated on its own for the FP32 forms: 183 instead
-// of 241 VGPRs, 3-4% faster at 32^3x64.
+// baryon_2pt.py) is instantiated on its own: for the FP32 forms 183 instead
+// of 241 VGPRs, 3-4% faster at 32^3x64; for FP64 it is what lets the staged
+// form fit two waves per SIMD.
 enum { TERMS_RUNTIME = 0, TERMS_NUCLEON = 1 };
 
 // SHARE: every request of the launch has propag_n == propag_i (a nucleon's
