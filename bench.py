@@ -77,6 +77,12 @@ class HipEvents:
     def destroy(self, e):
         self.h.hipEventDestroy(ctypes.c_void_p(e))
 
+    def record(self, e, stream):
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        rc = self.h.hipEventRecord(ctypes.c_void_p(e), ctypes.c_void_p(stream))
+        if rc != 0:
+            raise RuntimeError(f"hipEventRecord failed ({rc})")
+
 
 def make_propagators(torch, dims, seeds, alpha, device, t0=0):
     Lx, Ly, Lz, Lt = dims
@@ -149,6 +155,95 @@ def per_timeslice_error(got, want):
     return float((num / np.maximum(den, 1e-300)).max())
 
 
+CHANNEL_FLOOR = 1e-3  # channels below this fraction of their timeslice's largest entry are not rated
+
+
+def per_channel_error(got, want, floor=CHANNEL_FLOOR):
+    """Relative error of every channel (insertion, momentum, t) on its own,
+    |got - want| / |want|, over the channels with |want| at least `floor`
+    times the largest |want| of their timeslice (a channel far below that is
+    dominated by cancellation, and its relative error says nothing about the
+    contraction).  Returns max, median and the fraction of channels rated."""
+    a = np.abs(want)
+    keep = a >= floor * a.max(axis=(0, 1), keepdims=True)
+    rel = np.abs(got - want)[keep] / a[keep]
+    return {"max": float(rel.max()), "median": float(np.median(rel)), "rated": float(keep.mean()),
+            "floor": floor}
+
+
+def cpu_description():
+    """Host CPU as /proc/cpuinfo names it (model name, family/model/stepping,
+    logical CPUs)."""
+    info = {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ":" not in ln:
+                    if info:
+                        break
+                    continue
+                k, v = (x.strip() for x in ln.split(":", 1))
+                if k in ("vendor_id", "model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                    info[k] = v
+    except OSError:
+        pass
+    return {"model_name": info.get("model name", ""), "vendor": info.get("vendor_id", ""),
+            "family_model_stepping": "/".join(info.get(k, "?") for k in ("cpu family", "model", "stepping")),
+            "mhz": info.get("cpu MHz", ""), "logical_cpus": os.cpu_count()}
+
+
+class Probe:
+    """The HBM ceiling probes of contract_amd/lib/libcontract_probe.so."""
+
+    def __init__(self):
+        path = os.path.join(ROOT, "contract_amd", "lib", "libcontract_probe.so")
+        self.L = ctypes.CDLL(path)
+        P, S = ctypes.c_void_p, ctypes.c_size_t
+        self.L.probe_read.argtypes = [P, S, P, P]
+        self.L.probe_copy.argtypes = [P, P, S, P]
+        self.L.probe_read_blocks.restype = ctypes.c_int
+
+
+def hbm_ceilings(torch, hip, nbytes=4 << 30, reps=20):
+    """Measured HBM ceilings on this device, each timed with HIP events on
+    the stream its kernel runs on: a read-only stream (16-byte non-temporal
+    loads, eight in flight per thread) and a device copy (bytes read + bytes
+    written)."""
+    pr = Probe()
+    dev = torch.device("cuda:0")
+    src = torch.ones(nbytes // 8, dtype=torch.int64, device=dev)
+    dst = torch.empty_like(src)
+    sink = torch.empty(pr.L.probe_read_blocks(), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for name, fn, moved in (("read", lambda: pr.L.probe_read(src.data_ptr(), nbytes, sink.data_ptr(), stream),
+                             nbytes),
+                            ("copy", lambda: pr.L.probe_copy(dst.data_ptr(), src.data_ptr(), nbytes, stream),
+                             2 * nbytes)):
+        for _ in range(3):
+            if fn() != 0:
+                raise RuntimeError(f"probe_{name} launch failed")
+        torch.cuda.synchronize()
+        ev = [(hip.create(), hip.create()) for _ in range(reps)]
+        for a, b in ev:
+            hip.record(a, stream)
+            fn()
+            hip.record(b, stream)
+        ms = sorted(hip.elapsed_ms(a, b) for a, b in ev)
+        for a, b in ev:
+            hip.destroy(a)
+            hip.destroy(b)
+        med = ms[len(ms) // 2]
+        out[f"{name}_GBs"] = moved / (med * 1e-3) / 1e9
+        out[f"{name}_ms"] = med
+    if not torch.equal(dst[:4096], src[:4096]):
+        raise RuntimeError("probe_copy did not copy")
+    del src, dst, sink
+    torch.cuda.empty_cache()
+    out["bytes"] = nbytes
+    return out
+
+
 def cpu_baseline(torch, props, gpu_fn, target_s=10.0):
     """Time the FP64 oracle (oracle/, C + OpenMP over sites) on a bounded sample
     of the same workload -- the first nt timeslices, nt chosen for about
@@ -176,12 +271,8 @@ def cpu_baseline(torch, props, gpu_fn, target_s=10.0):
     oracle.set_num_threads(1)
     dt1, _, _, _ = run(1)
     oracle.set_num_threads(0)
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
+    cpu = cpu_description()
+    model = f"{cpu['model_name']} ({cpu['vendor']} family/model/stepping {cpu['family_model_stepping']})"
     return {
         "value": sites / dt,
         "unit": "sites/s",
@@ -252,12 +343,15 @@ def proton_32c64(torch, pc, dev, steps, warmup, hip, timer):
         if name == "fp64":
             ref = out.cpu().numpy()
         else:
-            res[name]["per_timeslice_rel_err_vs_fp64"] = per_timeslice_error(out.cpu().numpy(), ref)
+            got = out.cpu().numpy()
+            res[name]["per_timeslice_rel_err_vs_fp64"] = per_timeslice_error(got, ref)
+            res[name]["per_channel_rel_err_vs_fp64"] = per_channel_error(got, ref)
         if name != "fp32":
             res[name]["shared_u"] = shared_u(torch, pc, ps, dims, out, ws, steps, warmup, hip, timer, wall, kern,
                                              bps)
         del ps
     res["fp32_speedup"] = res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"]
+    res["fp32_kernel_speedup"] = res["fp64"]["kernel_ms"] / res["fp32"]["kernel_ms"]
     res["fp32_scaled_speedup"] = res["fp64"]["ms_per_step"] / res["fp32_scaled"]["ms_per_step"]
     res["fp32_scaled_kernel_speedup"] = res["fp64"]["kernel_ms"] / res["fp32_scaled"]["kernel_ms"]
     return res
@@ -361,13 +455,15 @@ def precision_study(torch, pc, dev, steps, warmup, hip, timer):
     t_oracle = time.perf_counter() - t0
     per_t = np.abs(got - want).max(axis=(0, 1)) / np.abs(want).max(axis=(0, 1))
     per_t_s = np.abs(got_s - want).max(axis=(0, 1)) / np.abs(want).max(axis=(0, 1))
+    got64 = out64.cpu().numpy()
     return {
         "workload": "sequential-source precision study, 32^3x64, exp(-0.8|t|), FP32 storage, Cg5, 16 insertions, p=0",
         "max_per_timeslice_rel_err": float(per_t.max()),
         "median_per_timeslice_rel_err": float(np.median(per_t)),
         "dynamic_range_decades": float(np.log10(np.abs(want[:, 0, 0]).max() / np.abs(want[:, 0, -1]).max())),
-        "fp64_gpu_vs_oracle_max_rel_err": float((np.abs(out64.cpu().numpy() - want).max(axis=(0, 1))
-                                                 / np.abs(want).max(axis=(0, 1))).max()),
+        "fp64_gpu_vs_oracle_max_rel_err": per_timeslice_error(got64, want),
+        "per_channel_rel_err": per_channel_error(got, want),
+        "fp64_gpu_vs_oracle_per_channel": per_channel_error(got64, want),
         "fp32_kernel_ms": kern32,
         "fp64_kernel_ms": kern64,
         "fp32_sites_per_s": V / (wall32 * 1e-3),
@@ -378,6 +474,7 @@ def precision_study(torch, pc, dev, steps, warmup, hip, timer):
         "fp64_achieved_GBs": BYTES_PER_SITE_FP64 * V / (kern64 * 1e-3) / 1e9,
         "fp32_scaled": {"max_per_timeslice_rel_err": float(per_t_s.max()),
                         "median_per_timeslice_rel_err": float(np.median(per_t_s)),
+                        "per_channel_rel_err": per_channel_error(got_s, want),
                         "kernel_ms": kern32s, "sites_per_s": V / (wall32s * 1e-3), "speedup": wall64 / wall32s,
                         "kernel_speedup": kern64 / kern32s if kern32s and kern64 else None},
         "oracle_s": t_oracle,
@@ -475,6 +572,43 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle, fi
     return res
 
 
+def threepoint_leg(torch, pc, dev, steps, warmup, hip, timer):
+    """Three-point step after the sequential source on 32^3x64: a forward
+    propagator F and a sequential propagator Sigma (FP64, synthetic, decaying
+    from t = 0 and from the sink t = 63), all 16 insertions, 7 momenta
+    (origin and unit axes) -> complex128[16][7][64]; FP64 and the FP32
+    site-scaled form.  Algorithmic bytes: 2 x 2304 B per site."""
+    dims = (32, 32, 32, 64)
+    V = 32 ** 3 * 64
+    f, = make_propagators(torch, dims, (5001,), ALPHA, dev)
+    s, = make_propagators(torch, dims, (5002,), ALPHA, dev, t0=63)
+    req = [(f, s, dims)]
+    ws = torch.empty(pc.threepoint_workspace_bytes(req, 7), dtype=torch.uint8, device=dev)
+    out = [torch.empty((16, 7, 64), dtype=torch.complex128, device=dev)]
+    res = {"workload": "three-point step 32^3x64, F and Sigma, 16 insertions, 7 momenta", "sites": V}
+    for name, r, bps in (("fp64", req, 2 * 2304), ("fp32_scaled", None, 2 * 1153)):
+        if r is None:
+            r = [(pc.toScaledFP32(f), pc.toScaledFP32(s), dims)]
+
+        def step():
+            pc.threePointProject(r, list(range(16)), MOM7, outs=out, workspace=ws)
+
+        wall, kern = time_steps(torch, step, steps, warmup, hip, timer)
+        gbs = bps * V / (kern * 1e-3) / 1e9
+        res[name] = {"ms_per_step": wall, "kernel_ms": kern, "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        if name == "fp64":
+            ref = out[0].cpu().numpy()
+        else:
+            got = out[0].cpu().numpy()
+            res[name]["per_timeslice_rel_err_vs_fp64"] = per_timeslice_error(got, ref)
+            res[name]["per_channel_rel_err_vs_fp64"] = per_channel_error(got, ref)
+        del r
+    res["summary"] = {"fp64_kernel_ms": _r(res["fp64"]["kernel_ms"]), "fp64_hbm_frac": _r(res["fp64"]["hbm_frac"]),
+                      "fp32_scaled_kernel_speedup": _r(res["fp64"]["kernel_ms"] / res["fp32_scaled"]["kernel_ms"]),
+                      "fp32_scaled_err_per_t": _r(res["fp32_scaled"]["per_timeslice_rel_err_vs_fp64"], 2)}
+    return res
+
+
 def abi_entry_points(torch, pc, u, d, u2, V, steps, warmup):
     """Every drop-in entry point once per step on the 24^3x64 FP64 propagators:
     ms per call and HBM GB/s of its algorithmic bytes (inputs read once,
@@ -541,7 +675,11 @@ def main():
     ap.add_argument("--grid-precision", default="fp64,fp32", help="precisions of the grid leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sweep")
     ap.add_argument("--only-serving", action="store_true", help="run only the serving-loop sweep")
-    ap.add_argument("--serving-rates", default="10,25,50,100,200", help="offered request rates (req/s)")
+    ap.add_argument("--serving-rates", default="10,25,50,100,200,300", help="offered request rates (req/s)")
+    ap.add_argument("--no-ceilings", action="store_true", help="skip the measured HBM ceilings")
+    ap.add_argument("--no-threepoint", action="store_true", help="skip the three-point leg")
+    ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
+                    help="file for the full per-leg results ('' to skip)")
     ap.add_argument("--serving-duration", type=float, default=3.0, help="seconds of arrivals per rate")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 2 GiB device copy, the reference kernel for scaling HBM counters")
@@ -597,6 +735,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    ceil = None if args.no_ceilings else hbm_ceilings(torch, hip)
     wall64, kern64 = time_steps(torch, step64, args.steps, args.warmup, hip, timer)
     ref64 = out64.cpu().numpy()
 
@@ -610,6 +749,7 @@ def main():
 
     wall32, kern32 = time_steps(torch, step32, args.steps, args.warmup, hip, timer)
     err32 = per_timeslice_error(out32.cpu().numpy(), ref64)
+    err32c = per_channel_error(out32.cpu().numpy(), ref64)
     del u_f, d_f, u2_f
 
     # FP32 site-scaled storage form of the same step (conversion timed apart)
@@ -627,6 +767,7 @@ def main():
 
     wall32s, kern32s = time_steps(torch, step32s, args.steps, args.warmup, hip, timer)
     err32s = per_timeslice_error(out32s.cpu().numpy(), ref64)
+    err32sc = per_channel_error(out32s.cpu().numpy(), ref64)
     del conv
 
     # drop-in per-site kernel (one insertion, one term), reference-style output
@@ -649,6 +790,10 @@ def main():
     if not args.no_grid:
         grid = meson_grid_batch(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer, not args.no_cpu)
 
+    threept = None
+    if not args.no_threepoint:
+        threept = threepoint_leg(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer)
+
     serving = None
     if not args.no_serving:
         torch.cuda.empty_cache()
@@ -662,6 +807,45 @@ def main():
 
     achieved = BYTES_PER_SITE_FP64 * V / (kern64 * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic("baryon_project_kernel<1,double,5,nucleon>")
+    detail = {
+        "fp32": {
+            "sites_per_s": V / (wall32 * 1e-3),
+            "ms_per_step": wall32,
+            "kernel_ms": kern32,
+            "achieved_GBs": BYTES_PER_SITE_FP32 * V / (kern32 * 1e-3) / 1e9,
+            "speedup_vs_fp64": wall64 / wall32,
+            "kernel_speedup_vs_fp64": kern64 / kern32,
+            "per_timeslice_rel_err_vs_fp64": err32,
+            "per_channel_rel_err_vs_fp64": err32c,
+        },
+        "fp32_scaled": {
+            "note": "FP32 site-scaled storage (contract_propagator_to_scaled_fp32): no per-site rescaling in "
+                    "the kernel; conversion of the three propagators timed apart",
+            "sites_per_s": V / (wall32s * 1e-3),
+            "ms_per_step": wall32s,
+            "kernel_ms": kern32s,
+            "achieved_GBs": (BYTES_PER_SITE_FP32 + 3) * V / (kern32s * 1e-3) / 1e9,
+            "speedup_vs_fp64": wall64 / wall32s,
+            "kernel_speedup_vs_fp64": kern64 / kern32s,
+            "per_timeslice_rel_err_vs_fp64": err32s,
+            "per_channel_rel_err_vs_fp64": err32sc,
+            "convert_ms_three_propagators": wall_conv,
+        },
+        "abi_baryon_two_point": {
+            "ms_per_call": wall_abi,
+            "sites_per_s": V / (wall_abi * 1e-3),
+            "achieved_GBs": (BYTES_PER_SITE_FP64 + 16) * V / (wall_abi * 1e-3) / 1e9,
+            "note": "one type (IM_JL_NK), one insertion, per-site output, like one reference call",
+            "entry_points": abi_all,
+        },
+        "precision_study": precision,
+        "proton_32c64": big,
+        "baryon_batch": batch,
+        "meson_grid_batch": grid,
+        "threepoint": threept,
+        "serving": serving,
+        "hbm_ceilings": ceil,
+    }
     result = {
         "metric": "baryon 2pt sites/s & % HBM roofline, FP32 vs FP64; p50/p99 request latency",
         "value": V / (wall64 * 1e-3),
@@ -678,11 +862,9 @@ def main():
         "config": {
             "workload": "proton 2pt, 24^3x64, u/d/u' FP64, Cg5, direct+exchange, 16 Gamma, 27 momenta "
                         "-> complex128[16][27][64]",
-            "lattice": list(DIMS),
             "sites": V,
             "insertions": 16,
             "momenta": 27,
-            "terms": ["IK_JL_NM", "IM_JL_NK"],
         },
         "roofline": {
             "bound": "hbm",
@@ -691,47 +873,81 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "traffic_source": traffic_src,
             "kernel": "baryon_project_kernel<1, double, 5, false, 1, false>",
             "kernel_ms": kern64,
             "bytes_per_site": BYTES_PER_SITE_FP64,
+            "measured_read_ceiling_GBs": ceil and round(ceil["read_GBs"], 1),
+            "measured_copy_ceiling_GBs": ceil and round(ceil["copy_GBs"], 1),
+            "frac_of_read_ceiling": ceil and achieved / ceil["read_GBs"],
         },
-        "fp32": {
-            "sites_per_s": V / (wall32 * 1e-3),
-            "ms_per_step": wall32,
-            "kernel_ms": kern32,
-            "achieved_GBs": BYTES_PER_SITE_FP32 * V / (kern32 * 1e-3) / 1e9,
-            "speedup_vs_fp64": wall64 / wall32,
-            "kernel_speedup_vs_fp64": kern64 / kern32,
-            "per_timeslice_rel_err_vs_fp64": err32,
-        },
-        "fp32_scaled": {
-            "note": "FP32 site-scaled storage (contract_propagator_to_scaled_fp32): no per-site rescaling in "
-                    "the kernel; conversion of the three propagators timed apart",
-            "sites_per_s": V / (wall32s * 1e-3),
-            "ms_per_step": wall32s,
-            "kernel_ms": kern32s,
-            "achieved_GBs": (BYTES_PER_SITE_FP32 + 3) * V / (kern32s * 1e-3) / 1e9,
-            "speedup_vs_fp64": wall64 / wall32s,
-            "kernel_speedup_vs_fp64": kern64 / kern32s,
-            "per_timeslice_rel_err_vs_fp64": err32s,
-            "convert_ms_three_propagators": wall_conv,
-        },
-        "abi_baryon_two_point": {
-            "ms_per_call": wall_abi,
-            "sites_per_s": V / (wall_abi * 1e-3),
-            "achieved_GBs": (BYTES_PER_SITE_FP64 + 16) * V / (wall_abi * 1e-3) / 1e9,
-            "note": "one type (IM_JL_NK), one insertion, per-site output, like one reference call",
-            "entry_points": abi_all,
-        },
-        "precision_study": precision,
-        "proton_32c64": big,
-        "baryon_batch": batch,
-        "meson_grid_batch": grid,
-        "serving": serving,
         "cpu_baseline": cpu,
+        "detail_file": None,
+        "north_star": north_star(kern64, achieved, kern32, kern32s, err32, err32s, err32c, err32sc, big, precision,
+                                 grid, batch, threept, serving),
     }
-    print(json.dumps(result))
+    if args.detail:
+        try:
+            os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
+            with open(args.detail, "w") as f:
+                json.dump(dict(result, detail=detail), f, indent=1)
+            result["detail_file"] = args.detail
+        except OSError as e:
+            print(f"bench: could not write {args.detail}: {e}", file=sys.stderr)
+    print(json.dumps(result, separators=(",", ":")))
+
+
+def _r(x, n=4):
+    """x to n significant digits (None stays None)."""
+    return None if x is None else float(f"{x:.{n}g}")
+
+
+def north_star(kern64, achieved, kern32, kern32s, err32, err32s, err32c, err32sc, big, precision, grid, batch,
+               threept, serving):
+    """The figures the targets are stated in, compact enough to survive the
+    tail of a log: FP64 kernel time and HBM fraction at 24^3x64 and 32^3x64,
+    the FP32 forms' kernel speedups and errors, the precision study's
+    errors, the meson grid's HBM fraction, and serving latency per rate."""
+    ns = {"fp64_24c64": {"kernel_ms": _r(kern64), "hbm_frac": _r(achieved / HBM_PEAK_GBS)},
+          "fp32_24c64": {"scaled_kernel_speedup": _r(kern64 / kern32s), "plain_kernel_speedup": _r(kern64 / kern32),
+                         "max_err_per_t": _r(max(err32, err32s), 2),
+                         "max_err_per_channel": _r(max(err32c["max"], err32sc["max"]), 2)}}
+    if big:
+        ns["fp64_32c64"] = {"kernel_ms": _r(big["fp64"]["kernel_ms"]), "hbm_frac": _r(big["fp64"]["hbm_frac"]),
+                            "target_hbm_frac": 0.70}
+        ns["fp32_32c64"] = {"scaled_kernel_speedup": _r(big["fp32_scaled_kernel_speedup"]),
+                            "plain_kernel_speedup": _r(big["fp32_kernel_speedup"]), "target_speedup": 1.8,
+                            "max_err_per_t": _r(max(big["fp32"]["per_timeslice_rel_err_vs_fp64"],
+                                                    big["fp32_scaled"]["per_timeslice_rel_err_vs_fp64"]), 2),
+                            "max_err_per_channel": _r(max(big["fp32"]["per_channel_rel_err_vs_fp64"]["max"],
+                                                          big["fp32_scaled"]["per_channel_rel_err_vs_fp64"]["max"]),
+                                                      2),
+                            "shared_u_fp64_kernel_ms": _r(big["fp64"]["shared_u"]["kernel_ms"])}
+    if precision:
+        ns["precision_study"] = {"max_err_per_t": _r(precision["max_per_timeslice_rel_err"], 2),
+                                 "scaled_max_err_per_t": _r(precision["fp32_scaled"]["max_per_timeslice_rel_err"], 2),
+                                 "max_err_per_channel": _r(precision["per_channel_rel_err"]["max"], 2),
+                                 "fp64_vs_oracle": _r(precision["fp64_gpu_vs_oracle_max_rel_err"], 2),
+                                 "decades": _r(precision["dynamic_range_decades"], 3),
+                                 "scaled_kernel_speedup": _r(precision["fp32_scaled"]["kernel_speedup"])}
+    if grid and "fp64" in grid:
+        ns["meson_grid"] = {"fp64_hbm_frac": _r(grid["fp64"]["hbm_frac"]),
+                            "fp64_kernel_ms": _r(grid["fp64"]["kernel_ms"])}
+        for k in ("fp32_scaled", "fp32"):
+            if k in grid:
+                ns["meson_grid"][k + "_speedup"] = _r(grid[k].get("speedup_vs_fp64"))
+    if batch:
+        ns["baryon_batch16"] = {"kernel_GBs": _r(batch["batched_kernel_GBs"] if "batched_kernel_GBs" in batch
+                                                 else batch["kernel_GBs"]),
+                                "speedup_vs_singles": _r(batch["speedup"])}
+    if threept:
+        ns["threepoint"] = threept.get("summary")
+    if serving:
+        ns["serving"] = {r: [_r(v["throughput_rps"]), _r(v["p50_ms"], 3), _r(v["p99_ms"], 3),
+                             _r(v["meson_batch_mean"], 3), _r(v["baryon_batch_mean"], 3)]
+                         for r, v in serving["rates"].items()}
+        ns["serving_columns"] = "rate: [served req/s, p50 ms, p99 ms, meson batch mean, baryon batch mean]"
+        ns["serving_saturation"] = serving.get("saturation")
+    return ns
 
 
 if __name__ == "__main__":

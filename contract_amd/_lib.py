@@ -41,6 +41,10 @@ EXT_SYMBOLS = (
     "contract_set_stream",
     "contract_meson_grid_workspace_bytes",
     "contract_meson_grid_project",
+    "contract_set_baryon_plan",
+    "contract_get_baryon_plan",
+    "contract_threepoint_workspace_bytes",
+    "contract_threepoint_project",
 )
 
 
@@ -101,6 +105,14 @@ def lib() -> ctypes.CDLL:
     L.contract_meson_grid_workspace_bytes.restype = S
     L.contract_meson_grid_project.argtypes = [P, I, I, P, I, P, P, S, P]
     L.contract_meson_grid_project.restype = I
+    L.contract_set_baryon_plan.argtypes = [P]
+    L.contract_set_baryon_plan.restype = I
+    L.contract_get_baryon_plan.argtypes = [P]
+    L.contract_get_baryon_plan.restype = I
+    L.contract_threepoint_workspace_bytes.argtypes = [P, I, I]
+    L.contract_threepoint_workspace_bytes.restype = S
+    L.contract_threepoint_project.argtypes = [P, I, I, P, I, P, I, P, S, P]
+    L.contract_threepoint_project.restype = I
     _lib = L
     return L
 

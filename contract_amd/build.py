@@ -61,9 +61,26 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     shutil.move(tmp, LIB)
+    build_probe()
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources")
+        print(f"built {LIB} from {len(sources)} sources, and {PROBE_LIB}")
     return LIB
+
+
+PROBE_SRC = os.path.join(HERE, "probe", "hbm_probe.hip")
+PROBE_LIB = os.path.join(LIBDIR, "libcontract_probe.so")
+
+
+def build_probe() -> str:
+    """The benchmark's HBM ceiling probes (read-only stream, device copy), a
+    library of their own: nothing in libcontract.so depends on them."""
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC, "-x", "hip", *HIP_FLAGS, "-shared", PROBE_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {PROBE_SRC}:\n{r.stdout}\n{r.stderr}")
+    shutil.move(tmp, PROBE_LIB)
+    return PROBE_LIB
 
 
 if __name__ == "__main__":

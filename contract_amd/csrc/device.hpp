@@ -15,22 +15,6 @@
 namespace contract_amd
 {
 
-#ifndef CONTRACT_COF_STREAM
-#define CONTRACT_COF_STREAM 1
-#endif
-
-// Scheduling fence: keeps the compiler from hoisting the next group's LDS
-// loads above the current group's arithmetic, which bounds the registers a
-// wave holds at once (needed to run two FP64 waves per SIMD).
-#ifndef CONTRACT_SCHED_FENCES
-#define CONTRACT_SCHED_FENCES 1
-#endif
-#if CONTRACT_SCHED_FENCES
-#define CONTRACT_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
-#else
-#define CONTRACT_SCHED_FENCE() ((void)0)
-#endif
-
 constexpr int NS = 4;
 constexpr int NC = 3;
 constexpr int SITE_ELEMS = 144; // [4][4][3][3] complex entries per site

@@ -77,14 +77,17 @@ std::vector<MomSlot> pair_momenta(const int *mom, int nmom)
   return slots;
 }
 
-constexpr size_t MIN_ITEM_SITES = 128;
-// 8 work items per CU: one item per resident wave of the two-wave staged
-// kernel (one round, no tail of half-empty rounds); 16 measured 3% slower
-// for the FP64 proton at 24^3x64 and 0-1.5% slower for the other forms
-#ifndef CONTRACT_ITEMS_PER_CU
-#define CONTRACT_ITEMS_PER_CU 8
-#endif
-constexpr size_t ITEMS_PER_CU = CONTRACT_ITEMS_PER_CU;
+// Work decomposition of the fused baryon kernels (contract_set_baryon_plan).
+// Defaults: 8 work items per CU, one item per resident wave of the two-wave
+// staged kernel (one round, no tail of half-empty rounds; 16 measured 3%
+// slower for the FP64 proton at 24^3x64 and 0-1.5% slower for the other
+// forms), and at least 128 sites (32 four-site steps) per item: a shorter
+// item spends a large part of its time on the prologue (phase tables, first
+// tile) and the partial store (16^3x32: 4.48 -> 4.76 TB/s single, 4.82 ->
+// 5.17 TB/s in a batch of 16).
+constexpr int DEFAULT_ITEMS_PER_CU = 8;
+constexpr int DEFAULT_MIN_ITEM_SITES = 128;
+static thread_local contract_baryon_plan g_baryon_plan = {DEFAULT_ITEMS_PER_CU, DEFAULT_MIN_ITEM_SITES};
 
 ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_item)
 {
@@ -96,16 +99,13 @@ ProjectPlan plan_project(const int dims[4], int order, int nmom, int sites_per_i
   if (sites_per_item > 0) {
     chunk = (size_t)sites_per_item;
   } else {
-    // enough work items to fill the chip several times over
-    const size_t target_items = (size_t)device_cu_count() * ITEMS_PER_CU;
+    // enough work items to fill the chip
+    const size_t target_items = (size_t)device_cu_count() * (size_t)g_baryon_plan.items_per_cu;
     size_t cps = (target_items + p.slices - 1) / p.slices;
     if (cps < 1) cps = 1;
     chunk = (p.slice_sites + cps - 1) / cps;
   }
-  // at least 32 four-site steps per item: a shorter item spends a large part
-  // of its time on the prologue (phase tables, first tile) and the partial
-  // store (16^3x32: 4.48 -> 4.76 TB/s single, 4.82 -> 5.17 TB/s in a batch of 16)
-  if (sites_per_item <= 0 && chunk < MIN_ITEM_SITES) chunk = MIN_ITEM_SITES;
+  if (sites_per_item <= 0 && chunk < (size_t)g_baryon_plan.min_item_sites) chunk = g_baryon_plan.min_item_sites;
   chunk = (chunk + SITES_PER_WAVE - 1) / SITES_PER_WAVE * SITES_PER_WAVE;
   if (chunk < SITES_PER_WAVE) chunk = SITES_PER_WAVE;
   p.chunk = (int)chunk;
@@ -244,6 +244,37 @@ const char *contract_last_error(void) { return g_last_error; }
 int contract_set_stream(void *stream)
 {
   g_abi_stream = static_cast<hipStream_t>(stream);
+  return CONTRACT_OK;
+}
+
+int contract_set_baryon_plan(const contract_baryon_plan *plan)
+{
+  if (!plan) {
+    g_baryon_plan = {DEFAULT_ITEMS_PER_CU, DEFAULT_MIN_ITEM_SITES};
+    return CONTRACT_OK;
+  }
+  contract_baryon_plan p = *plan;
+  if (p.items_per_cu == 0) p.items_per_cu = DEFAULT_ITEMS_PER_CU;
+  if (p.min_item_sites == 0) p.min_item_sites = DEFAULT_MIN_ITEM_SITES;
+  if (p.items_per_cu < 1 || p.items_per_cu > 256) {
+    set_last_error("items_per_cu = %d out of range 1..256", p.items_per_cu);
+    return CONTRACT_ERR_ARG;
+  }
+  if (p.min_item_sites < 4 || p.min_item_sites > (1 << 20)) {
+    set_last_error("min_item_sites = %d out of range 4..%d", p.min_item_sites, 1 << 20);
+    return CONTRACT_ERR_ARG;
+  }
+  g_baryon_plan = p;
+  return CONTRACT_OK;
+}
+
+int contract_get_baryon_plan(contract_baryon_plan *plan)
+{
+  if (!plan) {
+    set_last_error("plan is NULL");
+    return CONTRACT_ERR_ARG;
+  }
+  *plan = g_baryon_plan;
   return CONTRACT_OK;
 }
 

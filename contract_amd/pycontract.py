@@ -76,6 +76,67 @@ def _ptr(x) -> int:
     raise TypeError(f"expected a device pointer or tensor, got {type(x)}")
 
 
+def _nbytes(x) -> int:
+    if isinstance(x, ScaledPropagator):
+        x = x.data
+    return x.numel() * x.element_size()
+
+
+def _spans(*args):
+    """Raw-ABI argument check over (x, need_bytes, name) triples: a tensor
+    passed where the C entry point takes a device pointer must be contiguous
+    and hold at least `need` bytes (checked for every argument first), and all
+    such tensors must be on one GPU (an int pointer is the caller's
+    responsibility, as in C)."""
+    tensors = []
+    for x, need, what in args:
+        if x is None or isinstance(x, int):
+            continue
+        t = x.data if isinstance(x, ScaledPropagator) else x
+        if not hasattr(t, "is_contiguous"):
+            continue
+        if not t.is_contiguous():
+            raise ValueError(f"{what} must be contiguous")
+        if _nbytes(t) < need:
+            raise ValueError(f"{what} holds {_nbytes(t)} bytes, the call reads or writes {need}")
+        tensors.append(t)
+    _same_device(tensors, "the tensor arguments")
+
+
+def _same_device(tensors, what: str):
+    """Every tensor is on one GPU; checked after the size checks so that host
+    tests can reach those without a device."""
+    devs = []
+    for t in tensors:
+        if t is None:
+            continue
+        d = t.data.device if isinstance(t, ScaledPropagator) else t.device
+        if d.type != "cuda":
+            raise ValueError(f"{what} must be GPU tensors, got device {d}")
+        devs.append(d)
+    if len(set(devs)) > 1:
+        raise ValueError(f"{what} must share one GPU, got {sorted(set(str(d) for d in devs))}")
+
+
+def _check_out(out, shape, what: str = "out"):
+    import torch
+    if out.dtype != torch.complex128:
+        raise ValueError(f"{what} must be complex128, got {out.dtype}")
+    if tuple(out.shape) != tuple(shape):
+        raise ValueError(f"{what} has shape {tuple(out.shape)}, expected {tuple(shape)}")
+    if not out.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+
+
+def _check_workspace(ws, need: int = 0):
+    if ws is None:
+        return
+    if not ws.is_contiguous():
+        raise ValueError("workspace must be contiguous")
+    if need and _nbytes(ws) < need:
+        raise ValueError(f"workspace of {_nbytes(ws)} bytes is smaller than the {need} needed")
+
+
 # ---------------------------------------------------------------------------
 # C entry points (include/contract.h)
 # ---------------------------------------------------------------------------
@@ -84,7 +145,15 @@ def init(device: int) -> None:
     lib().init(int(device))
 
 
+SITE_BYTES_FP64 = 2304  # complex128 [4][4][3][3]
+
+
+def _props_args(volume, **props):
+    return [(x, SITE_BYTES_FP64 * int(volume), n) for n, x in props.items()]
+
+
 def meson_two_point(correl, propag_i, propag_j, volume: int, gamma_ij: int, gamma_kl: int) -> None:
+    _spans((correl, 16 * int(volume), "correl"), *_props_args(volume, propag_i=propag_i, propag_j=propag_j))
     lib().meson_two_point(_ptr(correl), _ptr(propag_i), _ptr(propag_j), int(volume), int(gamma_ij), int(gamma_kl))
 
 
@@ -95,20 +164,30 @@ def _ptr_array(correl) -> ctypes.Array:
     return (ctypes.c_void_p * 16)(*ptrs)
 
 
+def _span_all(correl, propag_i, propag_j, volume):
+    _spans(*[(c, 16 * int(volume), f"correl[{g}]") for g, c in enumerate(correl)],
+           *_props_args(volume, propag_i=propag_i, propag_j=propag_j))
+
+
 def meson_all_source_two_point(correl, propag_i, propag_j, volume: int, gamma_ij: int) -> None:
+    _span_all(correl, propag_i, propag_j, volume)
     lib().meson_all_source_two_point(_ptr_array(correl), _ptr(propag_i), _ptr(propag_j), int(volume), int(gamma_ij))
 
 
 def meson_all_sink_two_point(correl, propag_i, propag_j, volume: int, gamma_kl: int) -> None:
+    _span_all(correl, propag_i, propag_j, volume)
     lib().meson_all_sink_two_point(_ptr_array(correl), _ptr(propag_i), _ptr(propag_j), int(volume), int(gamma_kl))
 
 
 def baryon_diquark(diquark, propag_i, propag_j, volume: int, gamma_ij: int, gamma_kl: int) -> None:
+    _spans(*_props_args(volume, diquark=diquark, propag_i=propag_i, propag_j=propag_j))
     lib().baryon_diquark(_ptr(diquark), _ptr(propag_i), _ptr(propag_j), int(volume), int(gamma_ij), int(gamma_kl))
 
 
 def baryon_two_point(correl, propag_i, propag_j, propag_n, contract_type, volume: int, gamma_ij: int,
                      gamma_kl: int, gamma_mn: int) -> None:
+    _spans((correl, 16 * int(volume), "correl"),
+           *_props_args(volume, propag_i=propag_i, propag_j=propag_j, propag_n=propag_n))
     lib().baryon_two_point(_ptr(correl), _ptr(propag_i), _ptr(propag_j), _ptr(propag_n), int(contract_type),
                            int(volume), int(gamma_ij), int(gamma_kl), int(gamma_mn))
 
@@ -116,6 +195,8 @@ def baryon_two_point(correl, propag_i, propag_j, propag_n, contract_type, volume
 def baryon_general_two_point(correl, propag_i, propag_j, propag_n, contract_type, volume: int, gamma_ij: int,
                              gamma_kl: int, project_mn) -> None:
     proj = np.ascontiguousarray(np.asarray(project_mn, dtype=np.complex128).reshape(16))
+    _spans((correl, 16 * int(volume), "correl"),
+           *_props_args(volume, propag_i=propag_i, propag_j=propag_j, propag_n=propag_n))
     lib().baryon_general_two_point(_ptr(correl), _ptr(propag_i), _ptr(propag_j), _ptr(propag_n),
                                    int(contract_type), int(volume), int(gamma_ij), int(gamma_kl),
                                    proj.ctypes.data_as(ctypes.c_void_p))
@@ -123,6 +204,7 @@ def baryon_general_two_point(correl, propag_i, propag_j, propag_n, contract_type
 
 def baryon_sequential_two_point(propag_i, propag_j, propag_n, contract_type, sequential_type, volume: int,
                                 gamma_ij: int, gamma_kl: int, gamma_mn: int) -> None:
+    _spans(*_props_args(volume, propag_i=propag_i, propag_j=propag_j, propag_n=propag_n))
     lib().baryon_sequential_two_point(_ptr(propag_i), _ptr(propag_j), _ptr(propag_n), int(contract_type),
                                       int(sequential_type), int(volume), int(gamma_ij), int(gamma_kl),
                                       int(gamma_mn))
@@ -175,6 +257,7 @@ def toScaledFP32(propag, out=None, stream=None) -> ScaledPropagator:
     V = _volume(propag)
     if out is None:
         out = torch.empty(scaled_fp32_bytes(V), dtype=torch.uint8, device=propag.device)
+    _spans((out, V * 1153, "out"), (propag, V * SITE_BYTES_FP64, "propag"))
     rc = lib().contract_propagator_to_scaled_fp32(_ptr(out), _ptr(propag), V, 0 if stream is None else int(stream))
     if rc != 0:
         raise RuntimeError(f"contract_propagator_to_scaled_fp32 failed ({rc}): {last_error()}")
@@ -197,6 +280,11 @@ def _precision(props) -> int:
 
 def _volume(p) -> int:
     if isinstance(p, ScaledPropagator):
+        if not p.data.is_contiguous():
+            raise ValueError("a ScaledPropagator's data must be contiguous")
+        if _nbytes(p.data) < p.volume * 1153:
+            raise ValueError(f"ScaledPropagator of {p.volume} sites holds {_nbytes(p.data)} bytes, "
+                             f"the form needs {p.volume * 1153}")
         return p.volume
     if p.dtype.__str__() not in ("torch.complex128", "torch.complex64"):
         raise TypeError(f"propagators must be complex128/complex64 tensors, got {p.dtype}")
@@ -207,9 +295,39 @@ def _volume(p) -> int:
     return p.numel() // 144
 
 
+def _fp64_volume(*props) -> int:
+    """Site count of the drop-in helpers' complex128 propagators, which must
+    all hold the same number of sites on one GPU."""
+    import torch
+    for p in props:
+        if isinstance(p, ScaledPropagator) or p.dtype != torch.complex128:
+            raise TypeError("the drop-in entry points take complex128 propagators")
+    vols = [_volume(p) for p in props]
+    if len(set(vols)) != 1:
+        raise ValueError(f"propagators hold different site counts {vols}")
+    _same_device(props, "propagators")
+    return vols[0]
+
+
+def _volume_of_dims(dims) -> int:
+    d = [int(v) for v in dims]
+    if len(d) != 4:
+        raise ValueError(f"dims must be (Lx, Ly, Lz, Lt), got {dims}")
+    return d[0] * d[1] * d[2] * d[3]
+
+
+def _check_lattice_props(props, dims, what: str):
+    """Every propagator of a projected call holds exactly prod(dims) sites."""
+    V = _volume_of_dims(dims)
+    for k, p in enumerate(props):
+        if _volume(p) != V:
+            raise ValueError(f"{what}: propagator {k} holds {_volume(p)} sites, dims {tuple(dims)} give {V}")
+    return V
+
+
 def mesonTwoPoint(propag_i, propag_j, gamma_ij: int, gamma_kl: int):
     import torch
-    V = _volume(propag_i)
+    V = _fp64_volume(propag_i, propag_j)
     out = torch.empty(V, dtype=torch.complex128, device=propag_i.device)
     meson_two_point(out, propag_i, propag_j, V, gamma_ij, gamma_kl)
     return out
@@ -217,7 +335,7 @@ def mesonTwoPoint(propag_i, propag_j, gamma_ij: int, gamma_kl: int):
 
 def mesonAllSourceTwoPoint(propag_i, propag_j, gamma_ij: int):
     import torch
-    V = _volume(propag_i)
+    V = _fp64_volume(propag_i, propag_j)
     out = torch.empty((16, V), dtype=torch.complex128, device=propag_i.device)
     meson_all_source_two_point([out[g] for g in range(16)], propag_i, propag_j, V, gamma_ij)
     return out
@@ -225,7 +343,7 @@ def mesonAllSourceTwoPoint(propag_i, propag_j, gamma_ij: int):
 
 def mesonAllSinkTwoPoint(propag_i, propag_j, gamma_kl: int):
     import torch
-    V = _volume(propag_i)
+    V = _fp64_volume(propag_i, propag_j)
     out = torch.empty((16, V), dtype=torch.complex128, device=propag_i.device)
     meson_all_sink_two_point([out[g] for g in range(16)], propag_i, propag_j, V, gamma_kl)
     return out
@@ -233,7 +351,7 @@ def mesonAllSinkTwoPoint(propag_i, propag_j, gamma_kl: int):
 
 def baryonDiquark(propag_i, propag_j, gamma_ij: int, gamma_kl: int):
     import torch
-    V = _volume(propag_i)
+    V = _fp64_volume(propag_i, propag_j)
     out = torch.empty((V, 4, 4, 3, 3), dtype=torch.complex128, device=propag_i.device)
     baryon_diquark(out, propag_i, propag_j, V, gamma_ij, gamma_kl)
     return out
@@ -243,7 +361,7 @@ def baryonTwoPoint(propag_i, propag_j, propag_n, contract_type, gamma_ij: int, g
     """Per-site baryon correlator.  ``gamma_mn`` is a gamma index (sparse
     path) or a 4x4 projector matrix, row-major [m][n] (general path)."""
     import torch
-    V = _volume(propag_i)
+    V = _fp64_volume(propag_i, propag_j, propag_n)
     out = torch.empty(V, dtype=torch.complex128, device=propag_i.device)
     if isinstance(gamma_mn, (int, np.integer)):
         baryon_two_point(out, propag_i, propag_j, propag_n, contract_type, V, gamma_ij, gamma_kl, gamma_mn)
@@ -256,7 +374,7 @@ def baryonTwoPoint(propag_i, propag_j, propag_n, contract_type, gamma_ij: int, g
 def baryonSequentialTwoPoint(propag_i, propag_j, propag_n, contract_type, sequential_type, gamma_ij: int,
                              gamma_kl: int, gamma_mn: int):
     """Overwrites and returns the propagator selected by ``sequential_type``."""
-    V = _volume(propag_i)
+    V = _fp64_volume(propag_i, propag_j, propag_n)
     baryon_sequential_two_point(propag_i, propag_j, propag_n, contract_type, sequential_type, V, gamma_ij,
                                 gamma_kl, gamma_mn)
     return (propag_i, propag_j, propag_n)[int(sequential_type)]
@@ -314,6 +432,7 @@ def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, ga
     momenta are in units of 2pi / global extent."""
     import torch
     precision = _precision((propag_i, propag_j, propag_n))
+    _check_lattice_props((propag_i, propag_j, propag_n), dims, "baryonTwoPointProject")
     W = _projectors(insertions)
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
     types = np.ascontiguousarray([int(t) for t, _ in terms], dtype=np.int32)
@@ -321,6 +440,9 @@ def baryonTwoPointProject(propag_i, propag_j, propag_n, terms, gamma_ij: int, ga
     d = np.ascontiguousarray(dims, dtype=np.int32)
     if out is None:
         out = torch.empty((W.shape[0], mom.shape[0], int(d[3])), dtype=torch.complex128, device=propag_i.device)
+    _check_out(out, (W.shape[0], mom.shape[0], int(d[3])))
+    _check_workspace(workspace)
+    _same_device((propag_i, propag_j, propag_n, out, workspace), "propagators, output and workspace")
     ws_ptr, ws_bytes = 0, 0
     if workspace is not None:
         ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
@@ -387,6 +509,8 @@ def baryonTwoPointProjectBatch(requests, terms, gamma_ij: int, gamma_kl: int, in
     if not requests:
         return []
     precision = _precision([p for req in requests for p in req[:3]])
+    for k, req in enumerate(requests):
+        _check_lattice_props(req[:3], req[3], f"request {k}")
     W = _projectors(insertions)
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
     types = np.ascontiguousarray([int(t) for t, _ in terms], dtype=np.int32)
@@ -395,6 +519,13 @@ def baryonTwoPointProjectBatch(requests, terms, gamma_ij: int, gamma_kl: int, in
     if outs is None:
         outs = [torch.empty((W.shape[0], mom.shape[0], int(req[3][3])), dtype=torch.complex128,
                             device=req[0].device) for req in requests]
+    if len(outs) != len(requests):
+        raise ValueError(f"{len(outs)} outputs for {len(requests)} requests")
+    for k, (req, o) in enumerate(zip(requests, outs)):
+        _check_out(o, (W.shape[0], mom.shape[0], int(req[3][3])), f"outs[{k}]")
+    _check_workspace(workspace)
+    _same_device([p for req in requests for p in req[:3]] + list(outs) + [workspace],
+                 "propagators, outputs and workspace")
     for r, o in zip(arr, outs):
         r.out = _ptr(o)
     ws_ptr, ws_bytes = 0, 0
@@ -407,6 +538,87 @@ def baryonTwoPointProjectBatch(requests, terms, gamma_ij: int, gamma_kl: int, in
         mom.shape[0], ws_ptr, ws_bytes, 0 if stream is None else int(stream))
     if rc != 0:
         raise RuntimeError(f"contract_baryon_twopoint_project_batch failed ({rc}): {last_error()}")
+    return outs
+
+
+# ---------------------------------------------------------------------------
+# Three-point step after the sequential source (include/contract_ext.h)
+# ---------------------------------------------------------------------------
+
+class ThreePointRequest(ctypes.Structure):
+    """contract_threepoint_request."""
+    _fields_ = [("propag_fwd", ctypes.c_void_p), ("propag_seq", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("lattice", Sublattice)]
+
+
+def _threepoint_requests(requests):
+    """Request tuples (F, Sigma, dims[, site_order[, global_dims[, origin]]])."""
+    arr = (ThreePointRequest * len(requests))()
+    for r, req in zip(arr, requests):
+        r.propag_fwd, r.propag_seq = _ptr(req[0]), _ptr(req[1])
+        for mu in range(4):
+            r.lattice.dims[mu] = int(req[2][mu])
+        r.lattice.site_order = int(req[3]) if len(req) > 3 else LEXICOGRAPHIC
+        gl = req[4] if len(req) > 4 and req[4] is not None else ()
+        og = req[5] if len(req) > 5 and req[5] is not None else ()
+        for mu in range(4):
+            r.lattice.global_[mu] = int(gl[mu]) if mu < len(gl) else 0
+            r.lattice.origin[mu] = int(og[mu]) if mu < len(og) else 0
+    return arr
+
+
+def threepoint_workspace_bytes(requests, nmom: int) -> int:
+    arr = _threepoint_requests(requests)
+    return int(lib().contract_threepoint_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(arr), int(nmom)))
+
+
+def sequentialSource(seq):
+    """The solver source sigma of a sequential source seq = dC/dP_X (what
+    baryon_sequential_two_point leaves in P_X's buffer):
+    sigma[s1][s2](c1,c2) = g5[s1] conj(seq[s1][s2](c1,c2)).  Solving
+    D Sigma = sigma gives the sequential propagator of threePointProject."""
+    import torch
+    g5 = torch.tensor([1.0, 1.0, -1.0, -1.0], dtype=seq.real.dtype, device=seq.device)
+    return (seq.conj() * g5.view(1, 4, 1, 1, 1)).contiguous()
+
+
+def threePointProject(requests, insertions, momenta, outs=None, workspace=None, stream=None):
+    """Three-point functions after the sequential source: for each request
+    (F, Sigma, dims[, site_order[, global_dims[, origin]]]) return
+    out[len(insertions)][len(momenta)][Lt] (complex128, on the device) with
+    out[w][p][t] = sum_{x in t} exp(-i p.x) Tr[g5 Gamma_w F(x) Sigma(x)^dagger],
+    Gamma_w a gamma index or a 4x4 matrix.  F and Sigma are complex128,
+    complex64 or ScaledPropagator (one storage form per call)."""
+    import torch
+    if not requests:
+        return []
+    precision = _precision([p for req in requests for p in req[:2]])
+    for k, req in enumerate(requests):
+        _check_lattice_props(req[:2], req[2], f"request {k}")
+    G = _projectors(insertions)
+    mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
+    if outs is None:
+        outs = [torch.empty((G.shape[0], mom.shape[0], int(req[2][3])), dtype=torch.complex128, device=req[0].device)
+                for req in requests]
+    if len(outs) != len(requests):
+        raise ValueError(f"{len(outs)} outputs for {len(requests)} requests")
+    for k, (req, o) in enumerate(zip(requests, outs)):
+        _check_out(o, (G.shape[0], mom.shape[0], int(req[2][3])), f"outs[{k}]")
+    _check_workspace(workspace)
+    _same_device([p for req in requests for p in req[:2]] + list(outs) + [workspace],
+                 "propagators, outputs and workspace")
+    arr = _threepoint_requests(requests)
+    for r, o in zip(arr, outs):
+        r.out = _ptr(o)
+    ws_ptr, ws_bytes = 0, 0
+    if workspace is not None:
+        ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    vp = ctypes.c_void_p
+    rc = lib().contract_threepoint_project(ctypes.cast(arr, vp), len(arr), precision, G.ctypes.data_as(vp),
+                                           G.shape[0], mom.ctypes.data_as(vp), mom.shape[0], ws_ptr, ws_bytes,
+                                           0 if stream is None else int(stream))
+    if rc != 0:
+        raise RuntimeError(f"contract_threepoint_project failed ({rc}): {last_error()}")
     return outs
 
 
@@ -487,10 +699,19 @@ def mesonGridProject(requests, momenta, tuning=None, outs=None, workspace=None, 
     if not requests:
         return []
     precision = _precision([p for req in requests for p in req[:2]])
+    for k, req in enumerate(requests):
+        _check_lattice_props(req[:2], req[2], f"request {k}")
     mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
     if outs is None:
         outs = [torch.empty((16, 16, mom.shape[0], int(req[2][3])), dtype=torch.complex128, device=req[0].device)
                 for req in requests]
+    if len(outs) != len(requests):
+        raise ValueError(f"{len(outs)} outputs for {len(requests)} requests")
+    for k, (req, o) in enumerate(zip(requests, outs)):
+        _check_out(o, (16, 16, mom.shape[0], int(req[2][3])), f"outs[{k}]")
+    _check_workspace(workspace)
+    _same_device([p for req in requests for p in req[:2]] + list(outs) + [workspace],
+                 "propagators, outputs and workspace")
     arr = _request_array(requests, outs)
     t = _tuning(tuning)
     ws_ptr, ws_bytes = 0, 0
@@ -502,6 +723,50 @@ def mesonGridProject(requests, momenta, tuning=None, outs=None, workspace=None, 
     if rc != 0:
         raise RuntimeError(f"contract_meson_grid_project failed ({rc}): {last_error()}")
     return outs
+
+
+class BaryonPlan(ctypes.Structure):
+    """contract_baryon_plan; zero fields take the library default."""
+    _fields_ = [("items_per_cu", ctypes.c_int), ("min_item_sites", ctypes.c_int)]
+
+    def key(self):
+        return (self.items_per_cu, self.min_item_sites)
+
+
+def get_baryon_plan():
+    p = BaryonPlan()
+    if lib().contract_get_baryon_plan(ctypes.byref(p)) != 0:
+        raise RuntimeError(last_error())
+    return p.key()
+
+
+def set_baryon_plan(plan) -> None:
+    """(items_per_cu, min_item_sites) for this thread's fused baryon calls;
+    None restores the defaults."""
+    if plan is None:
+        rc = lib().contract_set_baryon_plan(None)
+    else:
+        p = BaryonPlan(*[int(v) for v in plan])
+        rc = lib().contract_set_baryon_plan(ctypes.byref(p))
+    if rc != 0:
+        raise ValueError(f"contract_set_baryon_plan failed: {last_error()}")
+
+
+class baryon_plan:
+    """Context manager: the fused baryon entry points called from this
+    thread use work plan ``plan`` = (items_per_cu, min_item_sites)."""
+
+    def __init__(self, plan):
+        self.plan = plan
+
+    def __enter__(self):
+        self.saved = get_baryon_plan()
+        set_baryon_plan(self.plan)
+        return self
+
+    def __exit__(self, *exc):
+        set_baryon_plan(self.saved)
+        return False
 
 
 class abi_stream:

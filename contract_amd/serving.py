@@ -2,15 +2,18 @@
 
 Requests (meson two-point grids, baryon two-points, diquarks) arrive over
 time and reference propagators by key.  The engine keeps propagators in an
-LRU pool with a byte cap, admits a waiting request when a launch slot is free
-and the pool can hold everything the request needs (its missing inputs and
-its outputs) without evicting anything in use, and batches continuously:
-every scheduler tick, the meson requests admitted together go through one
-batched persistent launch (contract_meson_grid_project), the baryon requests
-through one batched fused launch (contract_baryon_twopoint_project_batch),
-diquark requests get a launch each, and launches are spread over several
-streams so that work admitted while earlier work is still on the GPU
-overlaps with it.
+LRU pool with a byte cap, admits a waiting request when a launch slot of its
+kind is free and the pool can hold everything the request needs (its missing
+inputs and its outputs) without evicting anything in use, and batches
+continuously: each kind has a few launch slots, requests that arrive while
+those launches run wait in the queue, and when a slot frees the waiting
+requests of that kind and lattice go into the next launch together -- meson
+requests through one batched persistent launch (contract_meson_grid_project),
+baryon requests through one batched fused launch
+(contract_baryon_twopoint_project_batch), requests on the same propagators
+sharing one computation, diquark requests a launch each.  Launches are
+spread over several streams so that work admitted while earlier work is
+still on the GPU overlaps with it.
 A request completes when the event recorded after its last kernel has
 passed; its latency is completion minus arrival.
 
@@ -174,10 +177,33 @@ class PropagatorPool:
         self.peak = max(self.peak, self.used)
         return value
 
+    def needs_memory(self, keys: Sequence[Hashable], extra: Sequence[Hashable] = ()) -> bool:
+        """Would pinning `keys` plus new entries `extra` take memory (a missing
+        input to load, or an output to hold)?"""
+        return bool(extra) or any(k not in self.entries for k in keys)
+
+    def _recycle_same_kind(self, key):
+        """Evict the least recently used unpinned entry of the same type and
+        byte size as `key` and return its value: a released result's memory
+        then serves the next result of that size instead of a fresh
+        allocation (which costs tens of milliseconds for a 24 GB buffer)."""
+        for k in list(self.entries):
+            value, size, pins = self.entries[k]
+            if pins == 0 and size == key.nbytes and type(k) is type(key) and value is not None:
+                del self.entries[k]
+                self.used -= size
+                self.evictions += 1
+                return value
+        return None
+
     def insert(self, key, value):
         """Add a new pinned entry (a result, or None to reserve its bytes);
         returns a recycled value of the same size, or None."""
-        recycled = self._make_room(key.nbytes)
+        recycled = self._recycle_same_kind(key)
+        if recycled is None:
+            recycled = self._make_room(key.nbytes)
+        elif self.used + key.nbytes > self.cap:
+            self._make_room(key.nbytes)
         self.entries[key] = [value, key.nbytes, 1]
         self.used += key.nbytes
         self.peak = max(self.peak, self.used)
@@ -227,6 +253,7 @@ class Flight:
     event: object
     requests: List[Request]
     keep: list
+    kind: str = ""
 
 
 class Engine:
@@ -234,7 +261,16 @@ class Engine:
 
     def __init__(self, cap_bytes: int = 200 << 30, streams: int = 4, max_inflight: int = 8, device="cuda:0",
                  grid_tuning: Optional[Dict[str, Tuple[int, int, int]]] = None, poll_s: float = 5e-5,
-                 max_batch: int = 32, batch_baryons: bool = True):
+                 max_batch: int = 16, batch_baryons: bool = True, kind_slots: Optional[Dict[str, int]] = None,
+                 keep_results: bool = True, max_bypass_s: float = 0.05):
+        """max_inflight bounds the launches on the GPU at once; kind_slots
+        bounds them per request kind (default: 3 meson, 3 baryon, 2 diquark
+        launches), so that requests arriving while those run wait and go
+        into the next launch of their kind and lattice together (at most
+        max_batch requests per launch).  keep_results: diquark outputs stay
+        pinned in the pool until release_result() (False: they are unpinned
+        when the request completes and r.result is cleared, for callers that
+        only measure)."""
         import torch
         self.torch = torch
         self.device = torch.device(device)
@@ -243,12 +279,16 @@ class Engine:
         self.max_inflight = max_inflight
         self.max_batch = max_batch
         self.batch_baryons = batch_baryons
+        self.kind_slots = dict({"meson": 3, "baryon": 3, "diquark": 2}, **(kind_slots or {}))
+        self.keep_results = keep_results
+        self.max_bypass_s = max_bypass_s
         self.grid_tuning = dict(GRID_TUNING, **(grid_tuning or {}))
         self.poll_s = poll_s
         self.pool = PropagatorPool(cap_bytes, self._materialise)
         self._rr = 0
         self._spare: Dict[int, Entry] = {}
-        self.launches = {"meson_batches": 0, "meson_requests": 0, "baryon_batches": 0, "baryon": 0, "diquark": 0}
+        self.launches = {"meson_batches": 0, "meson_requests": 0, "meson_grids": 0, "baryon_batches": 0,
+                         "baryon": 0, "baryon_computed": 0, "diquark": 0}
 
     # -- pool ----------------------------------------------------------------
     def _materialise(self, key, recycled=None):
@@ -329,11 +369,21 @@ class Engine:
                     sub = [r for r in group if any(tuple(m) in moms for m in r.momenta)]
                     if not sub:
                         continue
-                    batch = [(inputs[r.rid][0], inputs[r.rid][1], r.dims, 0) for r in sub]
+                    # requests on the same propagator pair share one grid (it
+                    # holds every gamma pair; each request gathers its own)
+                    uniq: Dict[tuple, int] = {}
+                    for r in sub:
+                        uniq.setdefault(r.props, len(uniq))
+                    first = {}
+                    for r in sub:
+                        first.setdefault(r.props, r)
+                    batch = [(inputs[first[k].rid][0], inputs[first[k].rid][1], first[k].dims, 0) for k in uniq]
                     ws = torch.empty(pc.meson_grid_workspace_bytes(batch, len(moms), tuning), dtype=torch.uint8,
                                      device=self.device)
-                    outs = pc.mesonGridProject(batch, moms, tuning=tuning, workspace=ws, stream=stream.cuda_stream)
-                    keep += [ws] + outs
+                    grids = pc.mesonGridProject(batch, moms, tuning=tuning, workspace=ws, stream=stream.cuda_stream)
+                    keep += [ws] + grids
+                    self.launches["meson_grids"] += len(grids)
+                    outs = [grids[uniq[r.props]] for r in sub]
                     for r, o in zip(sub, outs):
                         # this pass's momenta of r: slot k in the pass -> position in r's result
                         pos = [(moms.index(tuple(m)), pi) for pi, m in enumerate(r.momenta) if tuple(m) in moms]
@@ -363,12 +413,20 @@ class Engine:
             by_dtype.setdefault(r.props[0].dtype, []).append(r)
         with torch.cuda.stream(stream):
             for dt, group in by_dtype.items():
-                batch = [(*self._inputs(r, stream), r.dims) for r in group]
+                # requests on the same propagators share one computation
+                uniq: Dict[tuple, int] = {}
+                first = {}
+                for r in group:
+                    uniq.setdefault((r.props, r.dims), len(uniq))
+                    first.setdefault((r.props, r.dims), r)
+                batch = [(*self._inputs(first[k], stream), first[k].dims) for k in uniq]
                 ws = torch.empty(pc.baryon_batch_workspace_bytes(batch, len(moms)), dtype=torch.uint8,
                                  device=self.device)
-                outs = pc.baryonTwoPointProjectBatch(batch, PROTON, CG5, CG5, gammas, moms, workspace=ws,
-                                                     stream=stream.cuda_stream)
-                keep += [ws] + outs
+                res = pc.baryonTwoPointProjectBatch(batch, PROTON, CG5, CG5, gammas, moms, workspace=ws,
+                                                    stream=stream.cuda_stream)
+                keep += [ws] + res
+                self.launches["baryon_computed"] += len(res)
+                outs = [res[uniq[(r.props, r.dims)]] for r in group]
                 for r, o in zip(group, outs):
                     gi = self._index([gammas.index(int(g)) for g in r.gammas])
                     mi = self._index([moms.index(tuple(m)) for m in r.momenta])
@@ -408,18 +466,85 @@ class Engine:
         extra = [r.out_key()] if r.kind == "diquark" else []
         return self.pool.fits(r.props, extra)
 
-    def _complete(self, f: Flight, now: float) -> None:
+    def _acquire_inputs(self, r: Request) -> None:
+        """Pin a request's inputs (one pin per listing, like release), the
+        resident ones first: materialising a missing input may evict
+        unpinned entries, and the request's own resident inputs, which
+        fits() counted as kept, must not be among them."""
+        keys = list(r.props)
+        resident = [k in self.pool for k in keys]
+        for k, res in zip(keys, resident):
+            if res:
+                self.pool.acquire(k)
+        for k, res in zip(keys, resident):
+            if not res:
+                self.pool.acquire(k)
+
+    def _complete(self, f: Flight, now: float, on_complete=None) -> None:
         for r in f.requests:
             r.done = now
             for k in r.props:
                 self.pool.release(k)
-            if r.kind == "diquark":
+            if r.kind == "diquark" and not self.keep_results:
                 self.pool.release(r.out_key())
+                r.result = None
+            if on_complete is not None:
+                on_complete(r)
 
-    def run(self, requests: Sequence[Request], realtime: bool = True) -> Dict[str, object]:
+    def result(self, r: Request):
+        """The result tensor of a completed request (a diquark's output is
+        held in the pool until release_result())."""
+        if r.kind == "diquark":
+            if r.result is None:
+                raise KeyError(f"request {r.rid}: the diquark output was not kept (keep_results=False)")
+            return self.pool.get(r.result).tensor
+        return r.result
+
+    def release_result(self, r: Request) -> None:
+        """Hand a diquark output's memory back to the pool."""
+        if r.kind == "diquark" and r.result is not None:
+            self.pool.release(r.result)
+            r.result = None
+
+    def warm_up(self, lattices: Sequence[Tuple[int, int, int, int]], dtype: str = "complex128") -> None:
+        """One request of every kind on every lattice, served and discarded:
+        the first launch of each kernel and the allocator's first blocks of
+        each size then fall outside the caller's timed window."""
+        reqs = []
+        rid = -1
+        for dims in lattices:
+            keys = [PropKey(tuple(dims), i, t0=(7 * i) % dims[3], dtype=dtype) for i in range(2)]
+            reqs.append(Request(rid, "meson", tuple(dims), (keys[0], keys[1]), [(5, 5)], MOM27))
+            reqs.append(Request(rid - 1, "baryon", tuple(dims), (keys[0], keys[1], keys[0]), [0, 8], MOM27))
+            reqs.append(Request(rid - 2, "diquark", tuple(dims), (keys[0], keys[1]), [(5, 5)], []))
+            rid -= 3
+        keep = self.keep_results
+        self.keep_results = False
+        try:
+            for r in reqs:
+                if self.pool.fits(r.props, [r.out_key()] if r.kind == "diquark" else []):
+                    self.run([r], realtime=False)
+        finally:
+            self.keep_results = keep
+        self.torch.cuda.synchronize()
+
+    def run(self, requests: Sequence[Request], realtime: bool = True,
+            on_complete: Optional[Callable[[Request], None]] = None) -> Dict[str, object]:
         """Serve `requests` (sorted by arrival time in seconds from the start).
         realtime=False admits everything at once (arrival times ignored for
-        admission but kept for latency)."""
+        admission but kept for latency).  on_complete(r) is called as each
+        request completes (its kernels have finished), e.g. to copy a result
+        out and release_result() it.
+
+        Admission walks the queue in arrival order.  Requests form launch
+        groups by kind and lattice (every diquark is a group of its own); a
+        request that opens a group needs a free launch slot of its kind, one
+        that joins an open group of this tick does not.  A request that finds
+        no slot waits for its kind's next launch.  The first request that
+        does not fit in the pool blocks everything behind it that would take
+        memory (a missing input or an output), so memory frees up for it;
+        requests on resident inputs that take none may pass it for up to
+        `max_bypass_s`, after which admission is strictly in order."""
         torch = self.torch
         for r in requests:
             need = sum(k.nbytes for k in set(r.props)) + (r.out_key().nbytes if r.kind == "diquark" else 0)
@@ -429,84 +554,105 @@ class Engine:
             for r in requests:
                 r.arrival = 0.0
         pending = deque(sorted(requests, key=lambda r: r.arrival))
-        queue: deque = deque()
+        queue: List[Request] = []
         flights: List[Flight] = []
         torch.cuda.synchronize()
         t_start = time.perf_counter()
-        batch_sizes = []
+        batch_sizes: Dict[str, List[int]] = {"meson": [], "baryon": []}
+        stats = {"host_launch_s": 0.0, "memory_blocked_ticks": 0, "idle_with_queue_s": 0.0, "bypassed": 0}
+        blocked_since: Dict[int, float] = {}
+        t_prev = 0.0
         while pending or queue or flights:
             now = time.perf_counter() - t_start
+            if queue and not flights:
+                stats["idle_with_queue_s"] += now - t_prev
+            t_prev = now
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
             progressed = False
             alive = []
             for f in flights:
                 if f.event.query():
-                    self._complete(f, time.perf_counter() - t_start)
+                    self._complete(f, time.perf_counter() - t_start, on_complete)
                     progressed = True
                 else:
                     alive.append(f)
             flights = alive
-            admitted: List[Request] = []
-            slots = self.max_inflight - len(flights)
-            nbatch = {"meson": 0, "baryon": 0}
-            while queue and slots > 0:
-                r = queue[0]
-                if (r.kind in nbatch and nbatch[r.kind] == self.max_batch) or not self._admissible(r):
-                    break
-                queue.popleft()
-                for k in r.props:
-                    self.pool.acquire(k)
+            busy = {k: sum(f.kind == k for f in flights) for k in KINDS}
+            free = {k: self.kind_slots[k] - busy[k] for k in KINDS}
+            total_free = self.max_inflight - len(flights)
+            groups: Dict[tuple, List[Request]] = {}
+            rest: List[Request] = []
+            head = None  # the first request blocked on memory
+            for r in queue:
+                alone = r.kind == "diquark" or (r.kind == "baryon" and not self.batch_baryons)
+                gkey = (r.kind, r.rid) if alone else (r.kind, r.dims)
+                joins = gkey in groups
+                if (joins and len(groups[gkey]) >= self.max_batch) or \
+                        (not joins and (free[r.kind] <= 0 or total_free <= 0)):
+                    rest.append(r)
+                    continue
+                extra = [r.out_key()] if r.kind == "diquark" else []
+                if head is not None and (self.pool.needs_memory(r.props, extra) or
+                                         now - blocked_since[head.rid] > self.max_bypass_s):
+                    rest.append(r)
+                    continue
+                if not self.pool.fits(r.props, extra):
+                    if head is None:
+                        head = r
+                        blocked_since.setdefault(r.rid, now)
+                    rest.append(r)
+                    continue
+                if head is not None:
+                    stats["bypassed"] += 1
+                blocked_since.pop(r.rid, None)
+                self._acquire_inputs(r)
                 if r.kind == "diquark":  # reserve the output's bytes
                     spare = self.pool.insert(r.out_key(), None)
                     if spare is not None:
                         self._spare[r.rid] = spare
                 r.admitted = time.perf_counter() - t_start
-                admitted.append(r)
-                if r.kind == "diquark" or (r.kind == "baryon" and not self.batch_baryons):
-                    slots -= 1
-                elif r.kind == "meson":  # one batched launch for all mesons of this tick
-                    nbatch["meson"] += 1
-                    slots -= nbatch["meson"] == 1
-                else:  # one batched launch per lattice for the baryons of this tick
-                    nbatch["baryon"] += 1
-                    slots -= not any(a.kind == "baryon" and a.dims == r.dims for a in admitted[:-1])
-            if admitted:
+                if not joins:
+                    groups[gkey] = []
+                    free[r.kind] -= 1
+                    total_free -= 1
+                groups[gkey].append(r)
+            queue = rest
+            if head is not None:
+                stats["memory_blocked_ticks"] += 1
+            if groups:
                 progressed = True
-                # mesons: one flight for the tick; baryons: one flight per
-                # lattice, so a batch on a small lattice completes (and
-                # releases its pinned propagators) without waiting for a large one
-                groups = [("meson", [r for r in admitted if r.kind == "meson"])]
-                if self.batch_baryons:
-                    for dims in sorted({r.dims for r in admitted if r.kind == "baryon"}):
-                        groups.append(("baryon", [r for r in admitted if r.kind == "baryon" and r.dims == dims]))
-                else:
-                    groups += [("baryon", [r]) for r in admitted if r.kind == "baryon"]
-                for kind, group in groups:
-                    if group:
-                        s = self._next_stream()
-                        keep = (self._launch_mesons if kind == "meson" else self._launch_baryons)(group, s)
-                        ev = torch.cuda.Event()
-                        ev.record(s)
-                        flights.append(Flight(ev, group, keep))
-                        if kind == "meson":
-                            batch_sizes.append(len(group))
-                for r in admitted:
-                    if r.kind != "diquark":
-                        continue
+                t_launch = time.perf_counter()
+                # one flight per group: a group on a small lattice completes,
+                # and releases its pinned propagators, without waiting for a
+                # large one
+                for (kind, _), group in groups.items():
                     s = self._next_stream()
-                    keep = self._launch_diquark(r, s)
+                    if kind == "meson":
+                        keep = self._launch_mesons(group, s)
+                    elif kind == "baryon":
+                        keep = self._launch_baryons(group, s)
+                    else:
+                        keep = self._launch_diquark(group[0], s)
                     ev = torch.cuda.Event()
                     ev.record(s)
-                    flights.append(Flight(ev, [r], keep))
+                    flights.append(Flight(ev, group, keep, kind))
+                    if kind in batch_sizes:
+                        batch_sizes[kind].append(len(group))
+                stats["host_launch_s"] += time.perf_counter() - t_launch
             if not progressed:
+                if queue and not flights and head is not None:
+                    raise PoolFull("the pool is held by results that were not released (release_result()) "
+                                   f"and cannot admit request {head.rid}")
                 wait = self.poll_s
                 if not queue and not flights and pending:
                     wait = max(0.0, min(pending[0].arrival - now, 0.01))
                 time.sleep(wait)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t_start
-        return summarise(requests, wall, self.pool, batch_sizes, self.launches)
+        out = summarise(requests, wall, self.pool, batch_sizes, self.launches)
+        out["scheduler"] = stats
+        return out
 
 
 def percentile(xs: Sequence[float], q: float) -> float:
@@ -532,7 +678,8 @@ def summarise(requests, wall, pool, batch_sizes, launches) -> Dict[str, object]:
         "per_kind": per_kind,
         "pool": {"cap_GB": pool.cap / 1e9, "peak_GB": pool.peak / 1e9, "hits": pool.hits, "misses": pool.misses,
                  "evictions": pool.evictions},
-        "meson_batch_mean": float(np.mean(batch_sizes)) if batch_sizes else 0.0,
+        "meson_batch_mean": float(np.mean(batch_sizes["meson"])) if batch_sizes["meson"] else 0.0,
+        "baryon_batch_mean": float(np.mean(batch_sizes["baryon"])) if batch_sizes["baryon"] else 0.0,
         "launches": dict(launches),
     }
 
@@ -585,12 +732,18 @@ def make_requests(rate: float, duration: float, seed: int, lattices=LATTICES, ca
 
 def sweep(rates: Sequence[float], duration: float, seed: int = 7, cap_gb: float = 200.0, streams: int = 4,
           max_inflight: int = 8, lattices=LATTICES, catalogue: int = 4, warm: bool = True,
-          batch_baryons: bool = True) -> Dict[str, object]:
+          batch_baryons: bool = True, kind_slots: Optional[Dict[str, int]] = None,
+          min_requests: int = 60) -> Dict[str, object]:
     """Serve one Poisson stream per rate on a shared engine (the pool stays warm
-    between rates) and report throughput and latency per rate."""
+    between rates) and report throughput and latency per rate.  Before the
+    first rate every kind of request runs once on every lattice (untimed),
+    so first launches and first allocations stay out of the windows."""
     eng = Engine(cap_bytes=int(cap_gb * 1e9), streams=streams, max_inflight=max_inflight,
-                 batch_baryons=batch_baryons)
-    if warm:  # load the catalogue, largest lattices last (most recently used)
+                 batch_baryons=batch_baryons, kind_slots=kind_slots, keep_results=False)
+    if warm:
+        eng.warm_up(lattices)
+        eng.launches = {k: 0 for k in eng.launches}
+        # load the catalogue, largest lattices last (most recently used)
         for dims in lattices:
             for i in range(catalogue):
                 k = PropKey(tuple(dims), i, t0=(7 * i) % dims[3])
@@ -601,16 +754,32 @@ def sweep(rates: Sequence[float], duration: float, seed: int = 7, cap_gb: float 
     res = {"rates": {}}
     rid = 0
     for k, rate in enumerate(rates):
-        reqs = make_requests(rate, duration, seed + k, lattices=lattices, catalogue=catalogue, first_rid=rid)
+        # at least min_requests per rate, so that p99 is not the single
+        # slowest request of a short window
+        span = max(duration, min_requests / rate)
+        reqs = make_requests(rate, span, seed + k, lattices=lattices, catalogue=catalogue, first_rid=rid)
         rid += len(reqs)
         print(f"serving: rate {rate} req/s, {len(reqs)} requests", file=sys.stderr, flush=True)
+        before = dict(eng.launches)
         res["rates"][str(rate)] = eng.run(reqs)
+        res["rates"][str(rate)]["launches"] = {n: eng.launches[n] - before[n] for n in eng.launches}
+    res["saturation"] = saturation(res["rates"])
     return res
+
+
+def saturation(per_rate: Dict[str, Dict[str, object]]) -> Dict[str, object]:
+    """The highest offered rate the engine still serves (throughput within
+    10% of the offered rate) and the largest throughput seen."""
+    ok = [float(r) for r, v in per_rate.items() if v["throughput_rps"] >= 0.9 * float(r)]
+    best = max(per_rate.items(), key=lambda kv: kv[1]["throughput_rps"])
+    return {"max_sustained_offered_rps": max(ok) if ok else None,
+            "peak_throughput_rps": best[1]["throughput_rps"], "at_offered_rps": float(best[0]),
+            "saturated": any(v["throughput_rps"] < 0.9 * float(r) for r, v in per_rate.items())}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--rates", default="10,25,50,100,200")
+    ap.add_argument("--rates", default="10,25,50,100,200,300,400")
     ap.add_argument("--duration", type=float, default=3.0)
     ap.add_argument("--cap-gb", type=float, default=200.0)
     ap.add_argument("--streams", type=int, default=4)

@@ -126,6 +126,38 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
                                            const int *momenta, int nmom, void *workspace, size_t workspace_bytes,
                                            void *stream);
 
+/* ---- three-point step after the sequential source ----------------------
+ * baryon_sequential_two_point (include/contract.h) leaves the sequential
+ * source seq_X = dC/dP_X of a two-point function in the buffer of P_X.  A
+ * solver turns it into the sequential propagator Sigma = D^-1 sigma, with
+ * source sigma(y)[s1][s2](c1,c2) = g5[s1] conj(seq_X(y)[s1][s2](c1,c2)).
+ * The three-point function of a current Gamma inserted at x on quark line
+ * X, with F the forward propagator of that line (F = P_X), is
+ *
+ *   out[w][p][t] = sum_{x in t} exp(-i 2pi p.x / L) Tr_s Tr_c[ g5 Gamma_w F(x) Sigma(x)^dagger ]
+ *
+ * (= meson_two_point(F, Sigma, volume, Gamma_w, 15) per site for a gamma
+ * index; any dense Gamma_w here), out being device complex128
+ * [nins][nmom][Lt] per request.  insertions: host double _Complex
+ * [nins][4][4] row-major (contract_gamma_matrix() for G(g)).  F and Sigma
+ * share a storage form (precision as above).  Up to 16 requests share a
+ * launch; momenta and insertions are shared; the workspace and the work
+ * plan (contract_set_baryon_plan) are those of the fused baryon entry
+ * points.  With D = 1 (Sigma = sigma) out equals the projected two-point
+ * function with P_X replaced by Gamma P_X. */
+typedef struct {
+  const void *propag_fwd; /* F, [V][4][4][3][3] */
+  const void *propag_seq; /* Sigma, [V][4][4][3][3] */
+  void *out;
+  contract_sublattice lattice;
+} contract_threepoint_request;
+
+size_t contract_threepoint_workspace_bytes(const contract_threepoint_request *reqs, int nreq, int nmom);
+
+int contract_threepoint_project(const contract_threepoint_request *reqs, int nreq, int precision,
+                                const double _Complex *insertions, int nins, const int *momenta, int nmom,
+                                void *workspace, size_t workspace_bytes, void *stream);
+
 /* ---- batched meson Gamma x Gamma' grids -------------------------------
  * One request = two propagators on one lattice.  For every request the full
  * 16 x 16 grid of meson_two_point(propag_i, propag_j, volume, g1, g2) (the
@@ -171,6 +203,20 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
  * thread (the reference launches them on the null stream, which stays the
  * default).  Lets a caller overlap reference-ABI work on its own streams. */
 int contract_set_stream(void *stream);
+
+/* Work decomposition of the fused baryon entry points above, for calls from
+ * this thread (including their workspace sizes): a lattice is cut into about
+ * items_per_cu work items per compute unit, each at least min_item_sites
+ * sites inside one timeslice.  A zero field takes the default (8 and 128);
+ * NULL restores both.  Results agree to rounding for any plan; for one plan
+ * they are bit-reproducible. */
+typedef struct {
+  int items_per_cu;   /* 1..256 */
+  int min_item_sites; /* 4..2^20 */
+} contract_baryon_plan;
+
+int contract_set_baryon_plan(const contract_baryon_plan *plan);
+int contract_get_baryon_plan(contract_baryon_plan *plan);
 
 /* Profiling hook: while set, every projected entry point called from this
  * thread records `start` (a hipEvent_t) on its stream right before its main

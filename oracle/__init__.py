@@ -56,6 +56,7 @@ def lib():
         L.oracle_site_coords.argtypes = [S, P, I, P]
         L.oracle_project.argtypes = [P, P, I, P, I, P, I, P, P]
         L.oracle_baryon_twopoint_project.argtypes = [P, P, P, P, P, P, I, I, I, P, I, P, I, P, I, P, P]
+        L.oracle_threepoint_project.argtypes = [P, P, P, P, I, P, I, P, I, P, P]
         L.oracle_num_threads.restype = I
         L.oracle_set_num_threads.argtypes = [I]
         L.oracle_set_num_threads.restype = None
@@ -227,6 +228,36 @@ def baryon_twopoint_project(propag_i, propag_j, propag_n, terms, gamma_ij: int, 
                                          mom.shape[0], None if gl is None else _ptr(gl),
                                          None if og is None else _ptr(og))
     return out
+
+
+def threepoint_project(fwd, seq, insertions, dims, momenta, order: int = 0, global_dims=None,
+                       origin=None) -> np.ndarray:
+    """Three-point step: out[w][p][t] = projection of
+    Tr_s Tr_c[g5 Gamma_w F(x) Sigma(x)^dagger] for the forward propagator F,
+    the sequential propagator Sigma and each insertion Gamma_w (a gamma index
+    g -> G(g), or a dense 4x4 matrix)."""
+    f, s = _prop(fwd), _prop(seq)
+    G = np.ascontiguousarray(np.stack([gamma(int(g)) if np.ndim(g) == 0 else np.asarray(g, np.complex128)
+                                       for g in insertions]))
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    mom = np.ascontiguousarray(momenta, dtype=np.int32).reshape(-1, 3)
+    out = np.zeros((G.shape[0], mom.shape[0], int(d[3])), np.complex128)
+    gl, og = _geom(global_dims), _geom(origin)
+    lib().oracle_threepoint_project(_ptr(out), _ptr(f), _ptr(s), _ptr(G), G.shape[0], _ptr(d), order, _ptr(mom),
+                                    mom.shape[0], None if gl is None else _ptr(gl), None if og is None else _ptr(og))
+    return out
+
+
+def sequential_propagator_unit_dirac(seq_source) -> np.ndarray:
+    """The sequential propagator Sigma = D^-1 sigma for the unit Dirac
+    operator D = 1, where sigma(y)[s1][s2](c1,c2) = g5[s1] conj(seq(y)[s1][s2](c1,c2))
+    is the source a solver is given for the sequential source seq = dC/dP_X
+    (g5-hermiticity S(y,x) = g5 S(x,y)^dagger g5 turns the backward line into
+    a forward solve).  With D = 1 the three-point function of insertion Gamma
+    equals the two-point function with P_X replaced by Gamma P_X."""
+    s = _prop(seq_source)
+    g5 = np.array([1.0, 1.0, -1.0, -1.0])
+    return np.ascontiguousarray(g5[None, :, None, None, None] * np.conj(s))
 
 
 def meson_grid_project(propag_i, propag_j, dims, momenta, order: int = 0, global_dims=None, origin=None) -> np.ndarray:

@@ -518,6 +518,49 @@ void oracle_baryon_twopoint_project(double *out, const double *propag_i, const d
   free(site);
 }
 
+/* ------------------------------------------------------------------ */
+/* Three-point step after the sequential source (new API).  The        */
+/* reference builds the sequential source (include/baryon_sequential.cuh:165-267, */
+/* lib/contract.cpp:46-63); the caller solves on it for the sequential */
+/* propagator Sigma and contracts Sigma with the forward propagator F  */
+/* of the quark line the current Gamma is inserted on.                 */
+/* ------------------------------------------------------------------ */
+
+/* c_w(x) = Tr_s Tr_c[ g5 Gamma_w F(x) Sigma(x)^dagger ]
+ *        = sum_{a,b,s,c,c'} (g5 Gamma_w)[a][b] F[b][s](c,c') conj(Sigma[a][s](c,c'))
+ * for each dense insertion Gamma_w ([4][4] row-major), projected as in
+ * oracle_project.  out is [nins][np][Lt]. */
+void oracle_threepoint_project(double *out, const double *fwd, const double *seq, const double *insertions, int nins,
+                               const int dims[4], int order, const int *mom, int np, const int *global,
+                               const int *origin)
+{
+  const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  cplx *site = (cplx *)calloc(V * (size_t)nins, sizeof(cplx));
+  const cplx *F = (const cplx *)fwd, *S = (const cplx *)seq, *Gm = (const cplx *)insertions;
+#pragma omp parallel for schedule(static)
+  for (long x = 0; x < (long)V; ++x) {
+    const cplx *f = F + x * SITE, *sg = S + x * SITE;
+    for (int w = 0; w < nins; ++w) {
+      cplx acc = cx(0, 0);
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+          const cplx g = Gm[(size_t)w * 16 + a * 4 + b];
+          const cplx A = cx(gamma5_diag(a) * g.re, gamma5_diag(a) * g.im);
+          if (!cnz(A)) continue;
+          cplx t = cx(0, 0);
+          for (int s = 0; s < 4; ++s) {
+            const cplx *fb = blk(f, b, s), *sa = blk(sg, a, s);
+            for (int cc = 0; cc < 9; ++cc) t = cadd(t, cmul(fb[cc], cconj(sa[cc])));
+          }
+          acc = cadd(acc, cmul(A, t));
+        }
+      site[(size_t)x * nins + w] = acc;
+    }
+  }
+  oracle_project(out, (const double *)site, nins, dims, order, mom, np, global, origin);
+  free(site);
+}
+
 int oracle_num_threads(void)
 {
 #ifdef _OPENMP

@@ -81,17 +81,58 @@ def test_mixed_stream_equals_alone_and_oracle(srv):
 
 
 def test_tight_pool_evicts_and_stays_correct(srv):
-    """A cap of four small propagators: requests wait for memory, the LRU
-    evicts between them, and every result still matches the oracle."""
+    """A cap of five small propagators: requests wait for memory, the LRU
+    evicts between them, diquark outputs are copied out and released as
+    their requests complete, and every result still matches the oracle."""
     torch, serving, cuda = srv
     lat = [(4, 4, 4, 8)]
-    reqs = serving.make_requests(2000.0, 0.01, seed=5, lattices=lat, catalogue=6, mix=(0.6, 0.4, 0.0),
+    reqs = serving.make_requests(2000.0, 0.012, seed=5, lattices=lat, catalogue=6, mix=(0.5, 0.3, 0.2),
                                  max_gammas=4, max_momenta=5)
-    cap = 4 * 512 * 2304
-    eng = serve(torch, serving, reqs, cap_bytes=cap, streams=3, max_inflight=4)
-    assert eng.pool.evictions > 0 and eng.pool.peak <= cap
+    assert {r.kind for r in reqs} == set(serving.KINDS)
+    cap = 5 * 512 * 2304
+    eng = serving.Engine(cap_bytes=cap, streams=3, max_inflight=4)
+    got = {}
+
+    def done(r):
+        got[r.rid] = eng.result(r).cpu().numpy().copy()
+        eng.release_result(r)
+
+    eng.run(reqs, realtime=False, on_complete=done)
+    assert eng.pool.evictions > 0 and eng.pool.peak <= cap and eng.pool.pinned_bytes() == 0
     for r in reqs:
-        assert rel_err(r.result.cpu().numpy(), oracle_result(torch, serving, r, cuda)) < 1e-12, (r.rid, r.kind)
+        assert rel_err(got[r.rid], oracle_result(torch, serving, r, cuda)) < 1e-12, (r.rid, r.kind)
+
+
+def test_unreleased_results_raise_instead_of_hanging(srv):
+    """Diquark outputs kept pinned and never released fill a small pool: the
+    loop reports it instead of waiting for memory that never frees."""
+    torch, serving, cuda = srv
+    k = [serving.PropKey((4, 4, 4, 8), i) for i in range(2)]
+    reqs = [serving.Request(i, "diquark", (4, 4, 4, 8), (k[0], k[1]), [(5, 5)], []) for i in range(4)]
+    eng = serving.Engine(cap_bytes=3 * 512 * 2304, streams=1, max_inflight=2)
+    with pytest.raises(serving.PoolFull):
+        eng.run(reqs, realtime=False)
+
+
+def test_coalesced_requests_match_alone(srv):
+    """Requests on the same propagators in one batch share one computation
+    and each still gets its own gammas and momenta, equal to serving it
+    alone."""
+    torch, serving, cuda = srv
+    dims = (4, 4, 4, 8)
+    k = [serving.PropKey(dims, i) for i in range(2)]
+    reqs = [serving.Request(0, "meson", dims, (k[0], k[1]), [(5, 5), (0, 8)], [(0, 0, 0), (1, 0, 0)]),
+            serving.Request(1, "meson", dims, (k[0], k[1]), [(3, 3)], [(0, -1, 0)]),
+            serving.Request(2, "baryon", dims, (k[0], k[1], k[0]), [0, 4], [(0, 0, 0)]),
+            serving.Request(3, "baryon", dims, (k[0], k[1], k[0]), [8], [(0, 0, 1), (1, 1, 1)])]
+    eng = serving.Engine(cap_bytes=1 << 30, streams=2, max_inflight=4)
+    eng.run(reqs, realtime=False)
+    assert eng.launches["meson_grids"] == 1 and eng.launches["baryon_computed"] == 1
+    for r in reqs:
+        alone = serving.Request(r.rid, r.kind, r.dims, r.props, r.gammas, r.momenta)
+        e1 = serve(torch, serving, [alone], cap_bytes=1 << 30, streams=1)
+        assert rel_err(r.result.cpu().numpy(), alone.result.cpu().numpy()) <= 1e-12
+        assert rel_err(r.result.cpu().numpy(), oracle_result(torch, serving, r, cuda)) < 1e-12
 
 
 def test_realtime_stream_reports_latency(srv):

@@ -145,3 +145,48 @@ def test_meson_momentum_groups_pair_plus_minus_k():
             for m in g:
                 neg = tuple(-v for v in m)
                 assert neg not in pick or neg in g
+
+
+def test_request_pins_resident_inputs_before_loading_missing_ones():
+    """Resident B, C, D in a pool of three, then a request on (A, B): loading
+    A must evict C (the least recently used entry the request does not
+    need), never the request's own resident input B."""
+    from contract_amd.serving import Engine, Request
+    p, made = pool(3)
+    a, b, c, d = (PropKey(SMALL, i) for i in range(4))
+    for k in (b, c, d):
+        p.acquire(k)
+        p.release(k)
+    eng = Engine.__new__(Engine)  # host-side scheduler state only
+    eng.pool = p
+    r = Request(0, "meson", SMALL, (a, b), [(5, 5)], [(0, 0, 0)])
+    assert p.fits(r.props)
+    eng._acquire_inputs(r)
+    assert a in p and b in p and c not in p and d in p
+    assert made == [b, c, d, a] and p.evictions == 1 and p.misses == 4
+    assert p.entries[a][2] == 1 and p.entries[b][2] == 1
+
+
+def test_repeated_input_is_pinned_once_per_listing():
+    """A nucleon request lists u twice (u, d, u): it is pinned twice and the
+    two releases of completion unpin it."""
+    from contract_amd.serving import Engine, Request
+    p, made = pool(4)
+    u, d = PropKey(SMALL, 0), PropKey(SMALL, 1)
+    eng = Engine.__new__(Engine)
+    eng.pool = p
+    r = Request(0, "baryon", SMALL, (u, d, u), [0], [(0, 0, 0)])
+    eng._acquire_inputs(r)
+    assert made == [u, d] and p.entries[u][2] == 2 and p.entries[d][2] == 1
+    for k in r.props:
+        p.release(k)
+    assert p.pinned_bytes() == 0
+
+
+def test_saturation_summary():
+    from contract_amd.serving import saturation
+    per = {"10.0": {"throughput_rps": 10.1}, "100.0": {"throughput_rps": 99.0},
+           "300.0": {"throughput_rps": 240.0}}
+    s = saturation(per)
+    assert s["max_sustained_offered_rps"] == 100.0 and s["saturated"]
+    assert s["peak_throughput_rps"] == 240.0 and s["at_offered_rps"] == 300.0
