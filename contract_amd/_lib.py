@@ -45,6 +45,8 @@ EXT_SYMBOLS = (
     "contract_get_baryon_plan",
     "contract_threepoint_workspace_bytes",
     "contract_threepoint_project",
+    "contract_noalloc_scratch_bytes",
+    "contract_baryon_twopoint_project_noalloc",
 )
 
 
@@ -113,6 +115,10 @@ def lib() -> ctypes.CDLL:
     L.contract_threepoint_workspace_bytes.restype = S
     L.contract_threepoint_project.argtypes = [P, I, I, P, I, P, I, P, S, P]
     L.contract_threepoint_project.restype = I
+    L.contract_noalloc_scratch_bytes.argtypes = []
+    L.contract_noalloc_scratch_bytes.restype = S
+    L.contract_baryon_twopoint_project_noalloc.argtypes = [P, P, P, P, I, P, P, I, I, I, P, I, P, I, P, P]
+    L.contract_baryon_twopoint_project_noalloc.restype = I
     _lib = L
     return L
 

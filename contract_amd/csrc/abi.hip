@@ -5,6 +5,7 @@
 // allocates, copies to __constant__ memory, or synchronises.
 
 #include <contract.h>
+#include <contract_ext.h>
 
 #include "baryon.hpp"
 #include "runtime.hpp"
@@ -43,7 +44,15 @@ SparseSpin g5spT(int g) { return sparse_of(mat_transpose(gamma5_times(g))); }
 
 extern "C" {
 
-void init(int device) { CONTRACT_HIP_CHECK(hipSetDevice(device)); }
+void init(int device)
+{
+  CONTRACT_HIP_CHECK(hipSetDevice(device));
+  // resolve this device's scratch address for the no-allocation entry point
+  if (contract_amd::noalloc_setup() != contract_amd::CONTRACT_OK) {
+    std::fprintf(stderr, "HIP error in init: %s\n", contract_last_error());
+    std::exit(-1);
+  }
+}
 
 void meson_two_point(void *correl, void *propag_i, void *propag_j, unsigned long volume, int gamma_ij, int gamma_kl)
 {

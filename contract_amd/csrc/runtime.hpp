@@ -59,6 +59,8 @@ enum : int {
 };
 
 void set_last_error(const char *fmt, ...);
+// Resolve the current device's scratch of the no-allocation entry point.
+int noalloc_setup();
 // Stream of the calling thread's drop-in ABI launches (contract_set_stream;
 // the null stream unless set).
 hipStream_t abi_stream();

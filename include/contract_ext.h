@@ -126,6 +126,26 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
                                            const int *momenta, int nmom, void *workspace, size_t workspace_bytes,
                                            void *stream);
 
+/* ---- for callers that cannot allocate ----------------------------------
+ * No counterpart in the reference.  For code under HIP graph capture, tight
+ * serving loops and harnesses that forbid device allocation: the fused
+ * baryon two-point of contract_baryon_twopoint_project_sub with its
+ * workspace in a fixed device array of the library (64 MiB per GPU, held
+ * until the process ends; contract_noalloc_scratch_bytes()) instead of a
+ * caller workspace or a stream-ordered allocation.  The call allocates no
+ * device memory.  A call needing more scratch than the array holds
+ * (contract_baryon_batch_workspace_bytes of its one request) fails with
+ * CONTRACT_ERR_WORKSPACE.  Every call on one GPU uses that GPU's one array:
+ * calls that overlap on one GPU (other streams or host threads) race; order
+ * them on one stream. */
+size_t contract_noalloc_scratch_bytes(void);
+
+int contract_baryon_twopoint_project_noalloc(void *out, const void *propag_i, const void *propag_j,
+                                             const void *propag_n, int precision, const int *contract_types,
+                                             const double _Complex *weights, int nterm, int gamma_ij, int gamma_kl,
+                                             const double _Complex *projectors, int nproj, const int *momenta,
+                                             int nmom, const contract_sublattice *lattice, void *stream);
+
 /* ---- three-point step after the sequential source ----------------------
  * baryon_sequential_two_point (include/contract.h) leaves the sequential
  * source seq_X = dC/dP_X of a two-point function in the buffer of P_X.  A
