@@ -112,11 +112,33 @@ def measured_traffic(kernel):
         return None, None
 
 
-def time_steps(torch, fn, steps, warmup, hip=None, timer=None):
+PREWARM_S = 0.25  # seconds of continuous load before each leg's warmup steps (--prewarm)
+
+
+def progress(leg):
+    """One line per benchmark leg on stderr (the JSON line goes to stdout)."""
+    print(f"bench: {leg}", file=sys.stderr, flush=True)
+
+
+def time_steps(torch, fn, steps, warmup, hip=None, timer=None, prewarm_s=None):
     """Run warmup + timed steps; returns (wall ms/step, kernel ms/step or None).
-    Python's garbage collector is paused over the timed steps (as timeit
-    does): a collection while the host enqueues sub-millisecond steps left
-    the GPU idle and doubled a step's wall time in some runs."""
+
+    Before the `warmup` steps the step runs back to back for `prewarm_s`
+    seconds: the part's power management boosts the clock after idle and
+    pulls it back during the first ~20 ms of heavy load (the 24^3x64 proton
+    kernel measured 0.94 ms for the first two dispatches after 10 ms idle,
+    up to 1.44 ms for the next ones, ~0.95-1.0 ms once settled), so a short
+    timed window that starts from idle measures the transient, not the
+    sustained rate.  Python's garbage collector is paused over the timed
+    steps (as timeit does): a collection while the host enqueues
+    sub-millisecond steps left the GPU idle and doubled a step's wall time in
+    some runs."""
+    prewarm_s = PREWARM_S if prewarm_s is None else prewarm_s
+    t_end = time.perf_counter() + prewarm_s
+    while time.perf_counter() < t_end:
+        for _ in range(4):  # a few steps in flight, so the queue ends with the prewarm
+            fn()
+        torch.cuda.synchronize()
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -657,6 +679,7 @@ def serving_leg(torch, args):
 
 
 def main():
+    global PREWARM_S
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
@@ -670,6 +693,7 @@ def main():
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched-baryon and 32^3x64 proton legs")
     ap.add_argument("--only-grid", action="store_true", help="run only the batched meson grid leg")
+    ap.add_argument("--only-threepoint", action="store_true", help="run only the three-point leg")
     ap.add_argument("--grid-tuning", default=None,
                     help="sites_per_item,tile_sites,blocks_per_cu for the grid leg instead of autotuning")
     ap.add_argument("--grid-precision", default="fp64,fp32", help="precisions of the grid leg")
@@ -677,6 +701,8 @@ def main():
     ap.add_argument("--only-serving", action="store_true", help="run only the serving-loop sweep")
     ap.add_argument("--serving-rates", default="10,25,50,100,200,300", help="offered request rates (req/s)")
     ap.add_argument("--no-ceilings", action="store_true", help="skip the measured HBM ceilings")
+    ap.add_argument("--prewarm", type=float, default=PREWARM_S,
+                    help="seconds each timed leg runs back to back before its warmup steps")
     ap.add_argument("--no-threepoint", action="store_true", help="skip the three-point leg")
     ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
                     help="file for the full per-leg results ('' to skip)")
@@ -689,6 +715,7 @@ def main():
               f"(requested --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
 
+    PREWARM_S = max(0.0, args.prewarm)
     import torch
     from contract_amd import pycontract as pc
 
@@ -708,6 +735,10 @@ def main():
     if args.only_batch:
         print(json.dumps({"baryon_batch": baryon_batch(torch, pc, dev, args.steps, args.warmup, hip, timer),
                           "proton_32c64": proton_32c64(torch, pc, dev, args.steps, args.warmup, hip, timer)}))
+        return
+
+    if args.only_threepoint:
+        print(json.dumps({"threepoint": threepoint_leg(torch, pc, dev, args.steps, args.warmup, hip, timer)}))
         return
 
     if args.only_grid:
@@ -735,7 +766,9 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    progress("hbm ceilings")
     ceil = None if args.no_ceilings else hbm_ceilings(torch, hip)
+    progress("24^3x64 proton FP64")
     wall64, kern64 = time_steps(torch, step64, args.steps, args.warmup, hip, timer)
     ref64 = out64.cpu().numpy()
 
@@ -777,32 +810,40 @@ def main():
         pc.baryon_two_point(correl, u, d, u2, 5, V, GAMMA_CG5, GAMMA_CG5, 8)
 
     wall_abi, _ = time_steps(torch, step_abi, args.steps, args.warmup)
+    progress("drop-in entry points")
     abi_all = abi_entry_points(torch, pc, u, d, u2, V, max(5, args.steps // 5), args.warmup)
 
     precision = None
     if not args.no_precision:
         del ws
+        progress("precision study")
         precision = precision_study(torch, pc, dev, args.steps, args.warmup, hip, timer)
+    progress("32^3x64 proton")
     big = None if args.no_big else proton_32c64(torch, pc, dev, args.steps, args.warmup, hip, timer)
+    progress("baryon batch")
     batch = None if args.no_batch else baryon_batch(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer)
 
     grid = None
     if not args.no_grid:
+        progress("meson grid batch")
         grid = meson_grid_batch(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer, not args.no_cpu)
 
     threept = None
     if not args.no_threepoint:
+        progress("three-point")
         threept = threepoint_leg(torch, pc, dev, min(args.steps, 20), args.warmup, hip, timer)
 
     serving = None
     if not args.no_serving:
         torch.cuda.empty_cache()
+        progress("serving sweep")
         serving = serving_leg(torch, args)
 
     cpu = None
     if not args.no_cpu:
         def gpu_sample(devp, dims):
             return pc.baryonTwoPointProject(*devp, TERMS, GAMMA_CG5, GAMMA_CG5, gammas, MOM27, dims)
+        progress("cpu baseline")
         cpu = cpu_baseline(torch, (u, d, u2), gpu_sample)
 
     achieved = BYTES_PER_SITE_FP64 * V / (kern64 * 1e-3) / 1e9
@@ -862,6 +903,7 @@ def main():
         "config": {
             "workload": "proton 2pt, 24^3x64, u/d/u' FP64, Cg5, direct+exchange, 16 Gamma, 27 momenta "
                         "-> complex128[16][27][64]",
+            "prewarm_s": PREWARM_S,
             "sites": V,
             "insertions": 16,
             "momenta": 27,

@@ -62,9 +62,48 @@ def build(verbose: bool = True) -> str:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     shutil.move(tmp, LIB)
     build_probe()
+    prefix = install()
+    build_c_check(prefix)
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources, and {PROBE_LIB}")
+        print(f"built {LIB} from {len(sources)} sources, {PROBE_LIB}, installed to {prefix}, built {C_CHECK}")
     return LIB
+
+
+DEFAULT_PREFIX = os.path.join(ROOT, "build", "install")
+
+
+def install(prefix: str = DEFAULT_PREFIX) -> str:
+    """Install layout of the reference's build (CMakeLists.txt:44-51):
+    PREFIX/lib/libcontract.so and PREFIX/include/{contract.h,
+    contract_enum.h, contract_ext.h}, so a binding generator or a C build
+    points -I/-L at one prefix."""
+    os.makedirs(os.path.join(prefix, "lib"), exist_ok=True)
+    os.makedirs(os.path.join(prefix, "include"), exist_ok=True)
+    shutil.copy2(LIB, os.path.join(prefix, "lib", "libcontract.so"))
+    for h in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))):
+        shutil.copy2(h, os.path.join(prefix, "include", os.path.basename(h)))
+    return prefix
+
+
+C_CHECK_SRC = os.path.join(ROOT, "tests", "c", "abi_check.c")
+C_CHECK = os.path.join(ROOT, "tests", "c", "_build", "abi_check")
+
+
+def build_c_check(prefix: str = DEFAULT_PREFIX) -> str:
+    """The C99 consumer of the installed library (tests/c/abi_check.c),
+    compiled with gcc against PREFIX/include and linked with -lcontract from
+    PREFIX/lib (run path relative to the binary, so the tree can move)."""
+    os.makedirs(os.path.dirname(C_CHECK), exist_ok=True)
+    rel = os.path.relpath(os.path.join(prefix, "lib"), os.path.dirname(C_CHECK))
+    cmd = [os.environ.get("CC", "gcc"), "-std=c99", "-O2", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(prefix, "include"), "-I", "/opt/rocm/include", C_CHECK_SRC, "-o", C_CHECK + ".tmp",
+           "-L", os.path.join(prefix, "lib"), "-lcontract", "-L", "/opt/rocm/lib", "-lamdhip64", "-lm",
+           f"-Wl,-rpath,$ORIGIN/{rel}:/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed for {C_CHECK_SRC}:\n{r.stdout}\n{r.stderr}")
+    shutil.move(C_CHECK + ".tmp", C_CHECK)
+    return C_CHECK
 
 
 PROBE_SRC = os.path.join(HERE, "probe", "hbm_probe.hip")
