@@ -47,6 +47,7 @@ EXT_SYMBOLS = (
     "contract_threepoint_project",
     "contract_noalloc_scratch_bytes",
     "contract_baryon_twopoint_project_noalloc",
+    "contract_baryon_sequential",
 )
 
 
@@ -119,6 +120,8 @@ def lib() -> ctypes.CDLL:
     L.contract_noalloc_scratch_bytes.restype = S
     L.contract_baryon_twopoint_project_noalloc.argtypes = [P, P, P, P, I, P, P, I, I, I, P, I, P, I, P, P]
     L.contract_baryon_twopoint_project_noalloc.restype = I
+    L.contract_baryon_sequential.argtypes = [P, P, P, I, I, I, S, I, I, I, P]
+    L.contract_baryon_sequential.restype = I
     _lib = L
     return L
 

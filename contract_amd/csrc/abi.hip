@@ -7,6 +7,8 @@
 #include <contract.h>
 #include <contract_ext.h>
 
+#include <cstring>
+
 #include "baryon.hpp"
 #include "runtime.hpp"
 #include "spin.hpp"
@@ -189,6 +191,58 @@ void baryon_sequential_two_point(void *propag_i, void *propag_j, void *propag_n,
   a.gmnT = spT(gamma_mn);
   launch_sequential((int)contract_type, target, a, abi_stream());
   check_launch();
+}
+
+int contract_baryon_sequential(void *propag_i, void *propag_j, void *propag_n, int precision, int contract_type,
+                               int sequential_type, size_t volume, int gamma_ij, int gamma_kl, int gamma_mn,
+                               void *stream)
+{
+  if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32_SCALED) {
+    set_last_error("precision %d is not CONTRACT_FP64 or CONTRACT_FP32_SCALED", precision);
+    return CONTRACT_ERR_ARG;
+  }
+  if (sequential_type < 0 || sequential_type > 2) {
+    set_last_error("sequential_type %d is not SEQUENTIAL_I, SEQUENTIAL_J or SEQUENTIAL_N", sequential_type);
+    return CONTRACT_ERR_ARG;
+  }
+  if (contract_type < 0 || contract_type > 5) {
+    set_last_error("contract_type %d is not a BaryonContractType", contract_type);
+    return CONTRACT_ERR_ARG;
+  }
+  if (gamma_ij < 0 || gamma_ij > 15 || gamma_kl < 0 || gamma_kl > 15 || gamma_mn < 0 || gamma_mn > 15) {
+    set_last_error("gamma_ij = %d / gamma_kl = %d / gamma_mn = %d out of range 0..15", gamma_ij, gamma_kl, gamma_mn);
+    return CONTRACT_ERR_ARG;
+  }
+  if (!propag_i || !propag_j || !propag_n) {
+    set_last_error("NULL propagator pointer");
+    return CONTRACT_ERR_ARG;
+  }
+  if (volume == 0) return CONTRACT_OK;
+  void *P[3] = {propag_i, propag_j, propag_n};
+  const int target = sequential_type;
+  SequentialArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in0 = P[target == 0 ? 1 : 0];
+  a.in1 = P[target == 2 ? 1 : 2];
+  a.out = P[target];
+  a.volume = volume;
+  a.gij = sp(gamma_ij);
+  a.gijT = spT(gamma_ij);
+  a.gkl = sp(gamma_kl);
+  a.gklT = spT(gamma_kl);
+  a.gmn = sp(gamma_mn);
+  a.gmnT = spT(gamma_mn);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == CONTRACT_FP64) {
+    launch_sequential(contract_type, target, a, s);
+  } else { // the site exponents follow the complex64 entries of every buffer
+    a.exp0 = static_cast<const signed char *>(a.in0) + volume * SCALED_SITE_BYTES;
+    a.exp1 = static_cast<const signed char *>(a.in1) + volume * SCALED_SITE_BYTES;
+    a.exp_out = static_cast<signed char *>(a.out) + volume * SCALED_SITE_BYTES;
+    launch_sequential_fp32_scaled(contract_type, target, a, s);
+  }
+  CONTRACT_EXT_HIP(hipGetLastError());
+  return CONTRACT_OK;
 }
 
 } // extern "C"

@@ -164,10 +164,17 @@ struct SequentialArgs {
   const void *in0, *in1; // the two propagators that stay, in (i, j, n) order
   void *out;             // buffer of the removed propagator (never read)
   size_t volume;
+  // FP32 site-scaled storage only: the site exponents of in0, in1 and out
+  const signed char *exp0, *exp1;
+  signed char *exp_out;
   // each spin matrix in both orientations: G (row r -> column, G[r][c]) and
   // G^T (row c -> r, G[r][c])
   SparseSpin gij, gijT, gkl, gklT, gmn, gmnT;
 };
 void launch_sequential(int type, int target, const SequentialArgs &a, hipStream_t stream);
+// Same on FP32 site-scaled propagators (complex64 mantissas + int8 site
+// exponents): FP32 arithmetic on the O(1) mantissas, the output written in
+// the same form with each site renormalised.
+void launch_sequential_fp32_scaled(int type, int target, const SequentialArgs &a, hipStream_t stream);
 
 } // namespace contract_amd

@@ -3,6 +3,8 @@
 // lane of a 16-lane site group, written back through LDS so the HBM stores
 // are coalesced like the loads.
 
+#include <type_traits>
+
 #include "baryon.hpp"
 #include "device.hpp"
 
@@ -87,10 +89,16 @@ __device__ __forceinline__ void seq_term(C (&acc)[9], const C *s0, const C *s1, 
   cof_acc(acc, y, z, w);
 }
 
-template <int T, int X>
+// SCALED: FP32 site-scaled storage in and out.  The staged mantissas are
+// O(1) (each site's largest component in [1, 2)), so the FP32 cofactor sums
+// cannot underflow; each output site is renormalised the same way, its
+// exponent byte e_out = e_0 + e_1 + (the sum's own binary exponent), clamped
+// to [-127, 127] like the converter's, the mantissas scaled by 2^(e_0 + e_1 -
+// e_out) (exact unless the clamp pushes them below FP32's normal range).
+template <int T, int X, bool SCALED = false>
 __global__ __launch_bounds__(WAVE) void sequential_kernel(SequentialArgs a)
 {
-  using C = double2;
+  using C = typename std::conditional<SCALED, float2, double2>::type;
   constexpr int SL = site_lds<C>();
   __shared__ C lds[2 * SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
@@ -165,6 +173,24 @@ __global__ __launch_bounds__(WAVE) void sequential_kernel(SequentialArgs a)
         seq_term<T, X>(acc, s0, s1, i, j, k, l, m, n, w);
       }
     }
+    if constexpr (SCALED) {
+      float mx = 0.0f;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) mx = fmaxf(mx, fmaxf(fabsf(acc[e].x), fabsf(acc[e].y)));
+#pragma unroll
+      for (int m = 8; m >= 1; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m, 64));
+      const size_t site = first + (size_t)(g < nsites ? g : nsites - 1); // padded lanes repeat the last site
+      const int ein = (int)a.exp0[site] + (int)a.exp1[site];
+      int eout = 0;
+      if (mx > 0.0f) {
+        eout = ein + ilogbf(mx);
+        eout = eout < -SCALED_EXP_MAX ? -SCALED_EXP_MAX : (eout > SCALED_EXP_MAX ? SCALED_EXP_MAX : eout);
+      }
+      const int sh = mx > 0.0f ? ein - eout : 0;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) acc[e] = make_float2(ldexpf(acc[e].x, sh), ldexpf(acc[e].y, sh));
+      if (q == 0 && g < nsites) a.exp_out[first + g] = (signed char)eout;
+    }
     __syncthreads();
     C *row = lds + g * SL + q * 9;
 #pragma unroll
@@ -174,29 +200,40 @@ __global__ __launch_bounds__(WAVE) void sequential_kernel(SequentialArgs a)
   });
 }
 
-template <int T>
+template <int T, bool SCALED>
 static void launch_seq_t(int target, const SequentialArgs &a, dim3 grid, hipStream_t stream)
 {
   switch (target) {
-  case 0: hipLaunchKernelGGL((sequential_kernel<T, 0>), grid, dim3(WAVE), 0, stream, a); break;
-  case 1: hipLaunchKernelGGL((sequential_kernel<T, 1>), grid, dim3(WAVE), 0, stream, a); break;
-  default: hipLaunchKernelGGL((sequential_kernel<T, 2>), grid, dim3(WAVE), 0, stream, a); break;
+  case 0: hipLaunchKernelGGL((sequential_kernel<T, 0, SCALED>), grid, dim3(WAVE), 0, stream, a); break;
+  case 1: hipLaunchKernelGGL((sequential_kernel<T, 1, SCALED>), grid, dim3(WAVE), 0, stream, a); break;
+  default: hipLaunchKernelGGL((sequential_kernel<T, 2, SCALED>), grid, dim3(WAVE), 0, stream, a); break;
   }
 }
 
-void launch_sequential(int type, int target, const SequentialArgs &a, hipStream_t stream)
+template <bool SCALED>
+static void launch_sequential_form(int type, int target, const SequentialArgs &a, hipStream_t stream)
 {
   const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
   if (ntiles == 0) return;
   const dim3 grid((unsigned)persistent_grid(ntiles, 8));
   switch (type) {
-  case 0: launch_seq_t<0>(target, a, grid, stream); break;
-  case 1: launch_seq_t<1>(target, a, grid, stream); break;
-  case 2: launch_seq_t<2>(target, a, grid, stream); break;
-  case 3: launch_seq_t<3>(target, a, grid, stream); break;
-  case 4: launch_seq_t<4>(target, a, grid, stream); break;
-  default: launch_seq_t<5>(target, a, grid, stream); break;
+  case 0: launch_seq_t<0, SCALED>(target, a, grid, stream); break;
+  case 1: launch_seq_t<1, SCALED>(target, a, grid, stream); break;
+  case 2: launch_seq_t<2, SCALED>(target, a, grid, stream); break;
+  case 3: launch_seq_t<3, SCALED>(target, a, grid, stream); break;
+  case 4: launch_seq_t<4, SCALED>(target, a, grid, stream); break;
+  default: launch_seq_t<5, SCALED>(target, a, grid, stream); break;
   }
+}
+
+void launch_sequential(int type, int target, const SequentialArgs &a, hipStream_t stream)
+{
+  launch_sequential_form<false>(type, target, a, stream);
+}
+
+void launch_sequential_fp32_scaled(int type, int target, const SequentialArgs &a, hipStream_t stream)
+{
+  launch_sequential_form<true>(type, target, a, stream);
 }
 
 } // namespace contract_amd

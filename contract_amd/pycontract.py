@@ -244,6 +244,16 @@ class ScaledPropagator:
         """The int8 [V] site exponents, a view."""
         return self.data[self.volume * 1152:].view(dtype=__import__("torch").int8)
 
+    def to_complex128(self):
+        """The represented propagator, entries x 2^e(x), as complex128 [V, 4, 4, 3, 3]."""
+        import torch
+        scale = torch.ldexp(torch.ones(self.volume, dtype=torch.float64, device=self.device),
+                            self.exponents().to(torch.int32))
+        return (self.entries().to(torch.complex128) * scale.view(-1, 1, 1, 1, 1)).contiguous()
+
+    def clone(self):
+        return ScaledPropagator(self.data.clone(), self.volume)
+
 
 def scaled_fp32_bytes(volume: int) -> int:
     return int(lib().contract_scaled_fp32_bytes(int(volume)))
@@ -576,10 +586,47 @@ def sequentialSource(seq):
     """The solver source sigma of a sequential source seq = dC/dP_X (what
     baryon_sequential_two_point leaves in P_X's buffer):
     sigma[s1][s2](c1,c2) = g5[s1] conj(seq[s1][s2](c1,c2)).  Solving
-    D Sigma = sigma gives the sequential propagator of threePointProject."""
+    D Sigma = sigma gives the sequential propagator of threePointProject.
+    A ScaledPropagator gives a ScaledPropagator (the transformation only
+    conjugates and flips signs, so the site exponents carry over)."""
     import torch
+    if isinstance(seq, ScaledPropagator):
+        out = seq.clone()
+        m = out.entries()
+        g5 = torch.tensor([1.0, 1.0, -1.0, -1.0], dtype=torch.float32, device=m.device)
+        m.copy_(m.conj() * g5.view(1, 4, 1, 1, 1))
+        return out
     g5 = torch.tensor([1.0, 1.0, -1.0, -1.0], dtype=seq.real.dtype, device=seq.device)
     return (seq.conj() * g5.view(1, 4, 1, 1, 1)).contiguous()
+
+
+def baryonSequential(propag_i, propag_j, propag_n, contract_type, sequential_type, gamma_ij: int, gamma_kl: int,
+                     gamma_mn: int, stream=None):
+    """contract_baryon_sequential: the sequential source seq_X = dC/dP_X into
+    the propagator selected by ``sequential_type`` (overwritten, never read),
+    which is returned.  All three complex128 tensors (FP64, as
+    baryon_sequential_two_point) or all three ScaledPropagator (FP32
+    site-scaled storage in and out)."""
+    props = (propag_i, propag_j, propag_n)
+    scaled = [isinstance(p, ScaledPropagator) for p in props]
+    if any(scaled) and not all(scaled):
+        raise TypeError("the three propagators must share one storage form")
+    if all(scaled):
+        V = props[0].volume
+        for p in props:
+            if _volume(p) != V:
+                raise ValueError(f"propagators hold different site counts {[q.volume for q in props]}")
+        _same_device(props, "propagators")
+        precision = CONTRACT_FP32_SCALED
+    else:
+        V = _fp64_volume(*props)
+        precision = CONTRACT_FP64
+    rc = lib().contract_baryon_sequential(_ptr(propag_i), _ptr(propag_j), _ptr(propag_n), precision,
+                                          int(contract_type), int(sequential_type), V, int(gamma_ij), int(gamma_kl),
+                                          int(gamma_mn), 0 if stream is None else int(stream))
+    if rc != 0:
+        raise RuntimeError(f"contract_baryon_sequential failed ({rc}): {last_error()}")
+    return props[int(sequential_type)]
 
 
 def threePointProject(requests, insertions, momenta, outs=None, workspace=None, stream=None):

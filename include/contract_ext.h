@@ -126,6 +126,20 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
                                            const int *momenta, int nmom, void *workspace, size_t workspace_bytes,
                                            void *stream);
 
+/* ---- sequential source in FP64 or FP32 site-scaled storage -------------
+ * baryon_sequential_two_point (include/contract.h; the reference's
+ * lib/contract.cpp:46-63) with a precision flag and an error code instead of
+ * exit(): CONTRACT_FP64 is the drop-in call on complex128 buffers;
+ * CONTRACT_FP32_SCALED takes all three buffers in the FP32 site-scaled form
+ * (contract_scaled_fp32_bytes(volume) bytes each) and writes the sequential
+ * source seq_X = dC/dP_X into the selected buffer in that form, mantissas
+ * and site exponents both (FP32 arithmetic on the stored mantissas; an
+ * output site below 2^-253 of scale loses its low bits).  The selected
+ * buffer is overwritten and never read. */
+int contract_baryon_sequential(void *propag_i, void *propag_j, void *propag_n, int precision, int contract_type,
+                               int sequential_type, size_t volume, int gamma_ij, int gamma_kl, int gamma_mn,
+                               void *stream);
+
 /* ---- for callers that cannot allocate ----------------------------------
  * No counterpart in the reference.  For code under HIP graph capture, tight
  * serving loops and harnesses that forbid device allocation: the fused
