@@ -500,7 +500,53 @@ def precision_study(torch, pc, dev, steps, warmup, hip, timer):
                         "kernel_ms": kern32s, "sites_per_s": V / (wall32s * 1e-3), "speedup": wall64 / wall32s,
                         "kernel_speedup": kern64 / kern32s if kern32s and kern64 else None},
         "oracle_s": t_oracle,
+        "sequential_chain": sequential_chain(torch, pc, dev, f32, steps, warmup),
     }
+
+
+def sequential_chain(torch, pc, dev, f32, steps, warmup):
+    """The sequential-source chain of the precision study with every
+    propagator in FP32 storage: the sequential source of the n quark
+    (IK_JL_NM, Cg5, insertion G(8)) from the two forward propagators, its
+    unit-Dirac sequential propagator, and the three-point step against the
+    forward n propagator for 16 insertions at p = 0 -- site-scaled FP32
+    throughout, against the same chain in FP64 on the same stored values
+    (the FP64 path is pinned to the oracle at 1e-12 in the tests)."""
+    dims = (32, 32, 32, 64)
+    V = 32 ** 3 * 64
+    mom = [(0, 0, 0)]
+    ins = list(range(16))
+    f64 = [p.to(torch.complex128) for p in f32]
+    fwd64 = f64[2].clone()
+    seq64 = f64[2]
+    pc.baryonSequential(f64[0], f64[1], seq64, 0, 2, GAMMA_CG5, GAMMA_CG5, 8)
+    want = pc.threePointProject([(fwd64, pc.sequentialSource(seq64), dims)], ins, mom)[0].cpu().numpy()
+    sc = [pc.toScaledFP32(p) for p in (f64[0], f64[1], fwd64)]
+    del f64, seq64
+    fwd32 = sc[2].clone()
+
+    def seq_step32():
+        pc.baryonSequential(sc[0], sc[1], sc[2], 0, 2, GAMMA_CG5, GAMMA_CG5, 8)
+
+    wall_seq32, _ = time_steps(torch, seq_step32, steps, warmup)
+    got = pc.threePointProject([(fwd32, pc.sequentialSource(sc[2]), dims)], ins, mom)[0].cpu().numpy()
+    del sc, fwd32
+    p64 = [torch.empty((V, 4, 4, 3, 3), dtype=torch.complex128, device=dev) for _ in range(3)]
+
+    def seq_step64():
+        pc.baryonSequential(p64[0], p64[1], p64[2], 0, 2, GAMMA_CG5, GAMMA_CG5, 8)
+
+    wall_seq64, _ = time_steps(torch, seq_step64, steps, warmup)
+    del p64, fwd64
+    torch.cuda.empty_cache()
+    return {"workload": "sequential source (n, IK_JL_NM, Cg5, G(8)) -> unit-Dirac sequential propagator -> "
+                        "three-point step, 16 insertions, p=0; FP32 site-scaled vs FP64",
+            "max_per_timeslice_rel_err": per_timeslice_error(got, want),
+            "per_channel_rel_err": per_channel_error(got, want),
+            "sequential_fp64_ms": wall_seq64, "sequential_fp32_scaled_ms": wall_seq32,
+            "sequential_GBs_fp64": 3 * 2304 * V / (wall_seq64 * 1e-3) / 1e9,
+            "sequential_GBs_fp32_scaled": 3 * 1153 * V / (wall_seq32 * 1e-3) / 1e9,
+            "sequential_fp32_speedup": wall_seq64 / wall_seq32}
 
 
 GRID_LATTICES = [(16, 16, 16, 32)] * 8 + [(24, 24, 24, 48)] * 8 + [(32, 32, 32, 64)] * 8
@@ -970,7 +1016,10 @@ def north_star(kern64, achieved, kern32, kern32s, err32, err32s, err32c, err32sc
                                  "max_err_per_channel": _r(precision["per_channel_rel_err"]["max"], 2),
                                  "fp64_vs_oracle": _r(precision["fp64_gpu_vs_oracle_max_rel_err"], 2),
                                  "decades": _r(precision["dynamic_range_decades"], 3),
-                                 "scaled_kernel_speedup": _r(precision["fp32_scaled"]["kernel_speedup"])}
+                                 "scaled_kernel_speedup": _r(precision["fp32_scaled"]["kernel_speedup"]),
+                                 "fp32_chain_err_per_t": _r(precision["sequential_chain"]["max_per_timeslice_rel_err"],
+                                                            2),
+                                 "fp32_sequential_speedup": _r(precision["sequential_chain"]["sequential_fp32_speedup"])}
     if grid and "fp64" in grid:
         ns["meson_grid"] = {"fp64_hbm_frac": _r(grid["fp64"]["hbm_frac"]),
                             "fp64_kernel_ms": _r(grid["fp64"]["kernel_ms"])}
