@@ -60,9 +60,9 @@ __device__ __forceinline__ int scale_block(float2 (&b)[9])
 //  KMFMA = false: on the vector ALUs, lane (i, j) of wave k forms K[i k][j l]
 //    for l = 0..3 from one P_i block and four P_j blocks read from LDS;
 //  KMFMA = true: on the matrix cores, wave w forms all of K for site w of each
-//    four-site step with 9 (FP64) or 12 (FP32) v_mfma 16x16x4 (3 colour
-//    chunks x 3 or 4 real products; A = P_i rows, B = P_j rows), each staged
-//    entry read from LDS once; K goes
+//    four-site step with 6 (FP64) or 8 (FP32) v_mfma 16x16x4 (2 colour
+//    chunks x 3 or 4 real products; A = P_i rows, B = P_j rows) and colour
+//    pair 8 on the vector ALUs, each staged entry read from LDS once; K goes
 //    through LDS once more to become the projection's B operand (4 sites x 16
 //    columns).  Needs FP64 or site-scaled FP32 storage: plain FP32 products of
 //    two steeply decaying propagators could underflow without a per-row scale.
@@ -241,12 +241,10 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
           // cancellation in Im costs FP32 accuracy)
           KAcc kr = KAcc{0, 0, 0, 0}, ki = KAcc{0, 0, 0, 0}, t3 = KAcc{0, 0, 0, 0};
 #pragma unroll
-          for (int ch = 0; ch < 3; ++ch) {
+          for (int ch = 0; ch < 2; ++ch) {
             // A[ik = q][kk = g] = P_i[q][c], B[kk = g][jl = q] = P_j[q][c], c = 4 ch + g
             const int cc = 4 * ch + g;
-            const int cl = cc < 9 ? cc : 8;
-            C xv = xs[q * 9 + cl], yv = ys[q * 9 + cl];
-            if (cc >= 9) xv = yv = czero<C>();
+            const C xv = xs[q * 9 + cc], yv = ys[q * 9 + cc];
             if constexpr (sizeof(Real) == 8) {
               kr = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.x, yv.x, kr, 0, 0, 0); // T1
               ki = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.y, yv.y, ki, 0, 0, 0); // T2
@@ -262,6 +260,18 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
             const KAcc t1 = kr, t2 = ki;
             kr = t1 + t2;
             ki = t3 - t1 + t2;
+          }
+          {
+            // colour pair 8 on the vector ALUs, in the MFMA output layout: a
+            // third colour chunk would multiply three zero pads per entry
+            const C yv = ys[q * 9 + 8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = sizeof(Real) == 8 ? g + 4 * v : 4 * g + v;
+              const C t = cfma_conj(cmake<C>(kr[v], ki[v]), xs[row * 9 + 8], yv);
+              kr[v] = t.x;
+              ki[v] = t.y;
+            }
           }
           int ex = 0;
           if constexpr (SCALED) {
