@@ -697,13 +697,18 @@ def make_requests(rate: float, duration: float, seed: int, lattices=LATTICES, ca
     """Poisson arrivals at `rate` per second over `duration` seconds: 50% meson,
     30% baryon, 20% diquark requests on a lattice drawn from `lattices`, each
     referencing propagators of a per-lattice catalogue of `catalogue` entries,
-    with 1..max_gammas gammas and 1..max_momenta momenta from {-1,0,1}^3."""
-    rng = np.random.default_rng(seed)
+    with 1..max_gammas gammas and 1..max_momenta momenta from {-1,0,1}^3.
+    The request contents and the arrival gaps come from two independent
+    streams of `seed`, and the gaps are unit-rate draws divided by `rate`:
+    for one seed, request i is the same request at every rate and only the
+    arrival times scale, so the points of a rate sweep differ in the rate
+    alone (common random numbers)."""
+    rng, arrivals = (np.random.default_rng(s) for s in np.random.SeedSequence(seed).spawn(2))
     out = []
     t = 0.0
     rid = first_rid
     while True:
-        t += rng.exponential(1.0 / rate)
+        t += arrivals.exponential(1.0) / rate
         if t > duration:
             break
         kind = KINDS[int(rng.choice(3, p=mix))]
@@ -753,11 +758,11 @@ def sweep(rates: Sequence[float], duration: float, seed: int = 7, cap_gb: float 
         eng.torch.cuda.synchronize()
     res = {"rates": {}}
     rid = 0
-    for k, rate in enumerate(rates):
+    for rate in rates:
         # at least min_requests per rate, so that p99 is not the single
         # slowest request of a short window
         span = max(duration, min_requests / rate)
-        reqs = make_requests(rate, span, seed + k, lattices=lattices, catalogue=catalogue, first_rid=rid)
+        reqs = make_requests(rate, span, seed, lattices=lattices, catalogue=catalogue, first_rid=rid)
         rid += len(reqs)
         print(f"serving: rate {rate} req/s, {len(reqs)} requests", file=sys.stderr, flush=True)
         before = dict(eng.launches)

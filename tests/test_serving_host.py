@@ -109,6 +109,20 @@ def test_request_stream_is_poisson_and_mixed():
         [(r.kind, r.dims, r.props, r.gammas, r.momenta, r.arrival) for r in reqs]
 
 
+def test_rates_share_the_request_sequence():
+    # one seed: request i is the same request at every rate, its arrival
+    # time scales as 1 / rate
+    slow = make_requests(10.0, 6.0, seed=7)
+    fast = make_requests(25.0, 6.0, seed=7)
+    n = min(len(slow), len(fast))
+    assert n > 40
+    for a, b in zip(slow[:n], fast[:n]):
+        assert (a.kind, a.dims, a.props, a.gammas, a.momenta) == (b.kind, b.dims, b.props, b.gammas, b.momenta)
+        assert abs(a.arrival * 10.0 - b.arrival * 25.0) < 1e-9
+    other = make_requests(10.0, 6.0, seed=8)
+    assert [r.kind for r in other[:n]] != [r.kind for r in slow[:n]]
+
+
 def test_catalogue_seeds_are_stable():
     k = PropKey((16, 16, 16, 32), 2, t0=14)
     assert k.seed == PropKey((16, 16, 16, 32), 2, t0=14).seed
