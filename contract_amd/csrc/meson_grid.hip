@@ -82,14 +82,16 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
   constexpr int PER_PROP = (CH + GRID_THREADS - 1) / GRID_THREADS; // chunks per thread and propagator
   constexpr int TILE16 = NS * T::LDS16;          // one propagator tile in LDS, 16-byte units
   __shared__ C lds[2][NS * SL];                  // [propagator]
-  // K of a four-site step, [slot][site][ik][jl]; two slots, so one barrier
-  // per step orders the writes of step n + 1 after the reads of step n.  Rows
+  // K of every site of the staged tile, [four-site group][site][ik][jl]:
+  // each wave forms K of its sites w, w + 4, ... of the tile, one barrier
+  // hands them over, then every wave projects its columns of all of them;
+  // the next tile's writes follow the barrier at the top of its step.  Rows
   // padded to 17 entries and sites to 16 * 17 + 1, so neither the MFMA-layout
   // writes (lane groups on rows g + 4v) nor the B-operand reads (lane groups
   // on sites) of a wave fall on one LDS bank
   constexpr int KROW = 17, KSITE = 16 * KROW + 1;
-  __shared__ double2 kbuf[KMFMA ? 2 : 1][KMFMA ? SITES_PER_WAVE : 1][KSITE];
-  int kslot = 0;
+  constexpr int NH = NS / SITES_PER_WAVE; // four-site groups per tile
+  __shared__ double2 kbuf[KMFMA ? NH : 1][KMFMA ? SITES_PER_WAVE : 1][KSITE];
   extern __shared__ double2 tab[];
   const int tid = threadIdx.x;
   const int w = tid >> 6, lane = tid & 63;
@@ -204,41 +206,19 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
       store();
       if (st + NS < last) load(st + NS);
       __syncthreads();
+      if constexpr (KMFMA) {
+        // traces of this wave's sites of the tile, into kbuf
 #pragma unroll
-      for (int h = 0; h < NS / SITES_PER_WAVE; ++h) {
-        const int ls = h * SITES_PER_WAVE + g; // this lane's site in the tile
-        // coordinates of site st + ls (sites past the item's end carry K = 0;
-        // their coordinates only need to stay inside the tables)
-        int xr = x0 + ls, y = y0, z = z0;
-        while (xr >= Lr) {
-          xr -= Lr;
-          if (++y == Ly) {
-            y = 0;
-            ++z;
-          }
-        }
-        z = z < Lz ? z : 0;
-        const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
-        double av[RT];
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          av[rt] = 0.0;
-          if (live[rt]) {
-            const double2 ph = cmul(cmul(tab[prow[rt] * (Lx + 1) + x], tab[offy + prow[rt] * (Ly + 1) + y]),
-                                    tab[offz + prow[rt] * (Lz + 1) + z]);
-            av[rt] = (q & 1) ? ph.y : ph.x;
-          }
-        }
-        if constexpr (KMFMA) {
-          using KAcc = typename std::conditional<sizeof(Real) == 8, dbl4, flt4>::type;
+        for (int h = 0; h < NH; ++h) {
           const int sk = h * SITES_PER_WAVE + w; // this wave's site
+          using KAcc = typename std::conditional<sizeof(Real) == 8, dbl4, flt4>::type;
           const C *xs = lds[0] + sk * SL, *ys = lds[1] + sk * SL;
           // FP64: three real products per colour chunk (Gauss), T1 = Xr Yr^T,
           // T2 = Xi Yi^T, T3 = (Xr + Xi)(Yr - Yi)^T, so K = X conj(Y)^T has
-          // Re = T1 + T2, Im = T3 - T1 + T2 (9 FP64 MFMAs per site, -4% on the
-          // batch: FP64 matrix issue is the scarcer resource).  FP32: four
-          // products (FP32 MFMA issue is twice as fast, and the Gauss form's
-          // cancellation in Im costs FP32 accuracy)
+          // Re = T1 + T2, Im = T3 - T1 + T2 (6 FP64 MFMAs per site: FP64
+          // matrix issue is the scarcer resource).  FP32: four products (FP32
+          // MFMA issue is twice as fast, and the Gauss form's cancellation in
+          // Im costs FP32 accuracy)
           KAcc kr = KAcc{0, 0, 0, 0}, ki = KAcc{0, 0, 0, 0}, t3 = KAcc{0, 0, 0, 0};
 #pragma unroll
           for (int ch = 0; ch < 2; ++ch) {
@@ -288,20 +268,47 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
               re = ldexp(re, ex);
               im = ldexp(im, ex);
             }
-            kbuf[kslot][w][row * KROW + q] = make_double2(re, im);
+            kbuf[h][w][row * KROW + q] = make_double2(re, im);
           }
-          __syncthreads();
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int h = 0; h < NS / SITES_PER_WAVE; ++h) {
+        const int ls = h * SITES_PER_WAVE + g; // this lane's site in the tile
+        // coordinates of site st + ls (sites past the item's end carry K = 0;
+        // their coordinates only need to stay inside the tables)
+        int xr = x0 + ls, y = y0, z = z0;
+        while (xr >= Lr) {
+          xr -= Lr;
+          if (++y == Ly) {
+            y = 0;
+            ++z;
+          }
+        }
+        z = z < Lz ? z : 0;
+        const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
+        double av[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          av[rt] = 0.0;
+          if (live[rt]) {
+            const double2 ph = cmul(cmul(tab[prow[rt] * (Lx + 1) + x], tab[offy + prow[rt] * (Ly + 1) + y]),
+                                    tab[offz + prow[rt] * (Lz + 1) + z]);
+            av[rt] = (q & 1) ? ph.y : ph.x;
+          }
+        }
+        if constexpr (KMFMA) {
           // rows ik = 4 i + k of this wave's k = w; B[site g][jl = q]
 #pragma unroll
           for (int i2 = 0; i2 < 4; ++i2) {
-            const double2 kv = kbuf[kslot][g][(4 * i2 + w) * KROW + q];
+            const double2 kv = kbuf[h][g][(4 * i2 + w) * KROW + q];
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
               acc_re[rt][i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kv.x, acc_re[rt][i2], 0, 0, 0);
               acc_im[rt][i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kv.y, acc_im[rt][i2], 0, 0, 0);
             }
           }
-          kslot ^= 1;
           continue;
         }
         const C *si = lds[0] + ls * SL, *sj = lds[1] + ls * SL;
