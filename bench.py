@@ -720,7 +720,16 @@ def serving_leg(torch, args):
     res = serving.sweep(rates, args.serving_duration, seed=7, cap_gb=200.0, streams=4, max_inflight=8)
     res["workload"] = ("Poisson arrivals, 50% meson / 30% baryon / 20% diquark, lattices 16^3x32, 24^3x64, "
                        "32^3x64, 48^3x96, 1-16 gammas, 1-27 momenta, LRU pool 200 GB, 4 streams")
+    gc.collect()  # the sweep's engine (its ~190 GB pool) sits in reference cycles until collected
     torch.cuda.empty_cache()
+    if args.serving_scaled_rates:
+        # the same request sequence against a pool of FP32 site-scaled
+        # propagators (packed from FP64 as they enter the pool)
+        progress("serving sweep, site-scaled FP32 pool")
+        rs = [float(r) for r in args.serving_scaled_rates.split(",")]
+        res["fp32_scaled"] = serving.sweep(rs, args.serving_duration, seed=7, cap_gb=200.0, streams=4,
+                                           max_inflight=8, dtype="fp32_scaled")
+        torch.cuda.empty_cache()
     return res
 
 
@@ -746,6 +755,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sweep")
     ap.add_argument("--only-serving", action="store_true", help="run only the serving-loop sweep")
     ap.add_argument("--serving-rates", default="10,25,50,100,200,300", help="offered request rates (req/s)")
+    ap.add_argument("--serving-scaled-rates", default="100,200,300,400",
+                    help="offered rates of the sweep against a site-scaled FP32 pool ('' to skip)")
     ap.add_argument("--no-ceilings", action="store_true", help="skip the measured HBM ceilings")
     ap.add_argument("--prewarm", type=float, default=PREWARM_S,
                     help="seconds each timed leg runs back to back before its warmup steps")
@@ -1038,6 +1049,12 @@ def north_star(kern64, achieved, kern32, kern32s, err32, err32s, err32c, err32sc
                          for r, v in serving["rates"].items()}
         ns["serving_columns"] = "rate: [served req/s, p50 ms, p99 ms, meson batch mean, baryon batch mean]"
         ns["serving_saturation"] = serving.get("saturation")
+        sc = serving.get("fp32_scaled")
+        if sc:
+            ns["serving_fp32_scaled"] = {r: [_r(v["throughput_rps"]), _r(v["p50_ms"], 3), _r(v["p99_ms"], 3),
+                                             _r(v["meson_batch_mean"], 3), _r(v["baryon_batch_mean"], 3)]
+                                         for r, v in sc["rates"].items()}
+            ns["serving_fp32_scaled_saturation"] = sc.get("saturation")
     return ns
 
 

@@ -48,6 +48,7 @@ EXT_SYMBOLS = (
     "contract_noalloc_scratch_bytes",
     "contract_baryon_twopoint_project_noalloc",
     "contract_baryon_sequential",
+    "contract_baryon_diquark",
 )
 
 
@@ -122,6 +123,8 @@ def lib() -> ctypes.CDLL:
     L.contract_baryon_twopoint_project_noalloc.restype = I
     L.contract_baryon_sequential.argtypes = [P, P, P, I, I, I, S, I, I, I, P]
     L.contract_baryon_sequential.restype = I
+    L.contract_baryon_diquark.argtypes = [P, P, P, I, S, I, I, P]
+    L.contract_baryon_diquark.restype = I
     _lib = L
     return L
 

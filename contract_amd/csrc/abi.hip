@@ -105,7 +105,7 @@ void baryon_diquark(void *diquark, void *propag_i, void *propag_j, size_t volume
   if (gamma_ij < 0 || gamma_ij > 15) return; // lib/baryon/diquark.cu:40: silent no-op
   abi_check_volume(volume);
   abi_check_gamma(gamma_kl, "gamma_kl");
-  DiquarkArgs a;
+  DiquarkArgs a = {};
   a.propag_i = propag_i;
   a.propag_j = propag_j;
   a.diquark = diquark;
@@ -191,6 +191,42 @@ void baryon_sequential_two_point(void *propag_i, void *propag_j, void *propag_n,
   a.gmnT = spT(gamma_mn);
   launch_sequential((int)contract_type, target, a, abi_stream());
   check_launch();
+}
+
+int contract_baryon_diquark(void *diquark, const void *propag_i, const void *propag_j, int precision, size_t volume,
+                            int gamma_ij, int gamma_kl, void *stream)
+{
+  if (precision != CONTRACT_FP64 && precision != CONTRACT_FP32_SCALED) {
+    set_last_error("precision %d is not CONTRACT_FP64 or CONTRACT_FP32_SCALED", precision);
+    return CONTRACT_ERR_ARG;
+  }
+  if (gamma_ij < 0 || gamma_ij > 15 || gamma_kl < 0 || gamma_kl > 15) {
+    set_last_error("gamma_ij = %d / gamma_kl = %d out of range 0..15", gamma_ij, gamma_kl);
+    return CONTRACT_ERR_ARG;
+  }
+  if (!diquark || !propag_i || !propag_j) {
+    set_last_error("NULL propagator pointer");
+    return CONTRACT_ERR_ARG;
+  }
+  if (volume == 0) return CONTRACT_OK;
+  DiquarkArgs a = {};
+  a.propag_i = propag_i;
+  a.propag_j = propag_j;
+  a.diquark = diquark;
+  a.volume = volume;
+  a.gij = sp(gamma_ij);
+  a.gklT = spT(gamma_kl);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == CONTRACT_FP64) {
+    launch_diquark(a, s);
+  } else { // the site exponents follow the complex64 entries of every buffer
+    a.exp_i = static_cast<const signed char *>(propag_i) + volume * SCALED_SITE_BYTES;
+    a.exp_j = static_cast<const signed char *>(propag_j) + volume * SCALED_SITE_BYTES;
+    a.exp_out = static_cast<signed char *>(diquark) + volume * SCALED_SITE_BYTES;
+    launch_diquark_fp32_scaled(a, s);
+  }
+  CONTRACT_EXT_HIP(hipGetLastError());
+  return CONTRACT_OK;
 }
 
 int contract_baryon_sequential(void *propag_i, void *propag_j, void *propag_n, int precision, int contract_type,

@@ -156,8 +156,13 @@ struct DiquarkArgs {
   void *diquark;
   size_t volume;
   SparseSpin gij, gklT;
+  // FP32 site-scaled storage only: the site exponents of P_i, P_j and the output
+  const signed char *exp_i, *exp_j;
+  signed char *exp_out;
 };
 void launch_diquark(const DiquarkArgs &a, hipStream_t stream);
+// Same on FP32 site-scaled propagators, the output in that form too.
+void launch_diquark_fp32_scaled(const DiquarkArgs &a, hipStream_t stream);
 
 // ---- sequential ---------------------------------------------------------
 struct SequentialArgs {

@@ -11,11 +11,44 @@
 namespace contract_amd
 {
 
+// FP32 site-scaled output (both kernels below): the staged mantissas are
+// O(1) (each site's largest component in [1, 2)), so the FP32 cofactor sums
+// cannot underflow; each output site is renormalised the same way, its
+// exponent byte e_out = e_0 + e_1 + (the sum's own binary exponent), clamped
+// to [-127, 127] like the converter's, the mantissas scaled by 2^(e_0 + e_1 -
+// e_out) (exact unless the clamp pushes them below FP32's normal range).
+// acc: this lane's 9 entries of its site's [16][9]; lane q = 0 of the site
+// group writes e_out.
+__device__ __forceinline__ void renormalise_output_site(float2 (&acc)[9], const signed char *e0,
+                                                        const signed char *e1, signed char *eo, size_t first,
+                                                        int nsites, int g, int q)
+{
+  float mx = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) mx = fmaxf(mx, fmaxf(fabsf(acc[e].x), fabsf(acc[e].y)));
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m, 64));
+  const size_t site = first + (size_t)(g < nsites ? g : nsites - 1); // padded lanes repeat the last site
+  const int ein = (int)e0[site] + (int)e1[site];
+  int eout = 0;
+  if (mx > 0.0f) {
+    eout = ein + ilogbf(mx);
+    eout = eout < -SCALED_EXP_MAX ? -SCALED_EXP_MAX : (eout > SCALED_EXP_MAX ? SCALED_EXP_MAX : eout);
+  }
+  const int sh = mx > 0.0f ? ein - eout : 0;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) acc[e] = make_float2(ldexpf(acc[e].x, sh), ldexpf(acc[e].y, sh));
+  if (q == 0 && g < nsites) eo[first + g] = (signed char)eout;
+}
+
 // D[m,l](a,d) = g5[m] g5[l] conj( sum_i Gij[i,j] Gkl[k,l] cof(P_i[i,k], P_j[j,m])[a,d] )
 // with j = col_ij(i), k = col_kl^T(l)  (include/diquark.cuh:32-58); lane q = m*4 + l.
+// SCALED: FP32 site-scaled storage in and out.
+template <bool SCALED>
 __global__ __launch_bounds__(WAVE) void diquark_kernel(DiquarkArgs a)
 {
-  using C = double2;
+  using C = typename std::conditional<SCALED, float2, double2>::type;
+  using R = decltype(C::x);
   constexpr int SL = site_lds<C>();
   __shared__ C lds[2 * SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
@@ -23,7 +56,7 @@ __global__ __launch_bounds__(WAVE) void diquark_kernel(DiquarkArgs a)
   const int m = q >> 2, l = q & 3;
   const int k = sparse_col(a.gklT, l);
   const C gl = sparse_val<C>(a.gklT, l);
-  const double s5 = ((m >> 1) ^ (l >> 1)) ? -1.0 : 1.0; // g5[m] g5[l]
+  const R s5 = ((m >> 1) ^ (l >> 1)) ? (R)-1 : (R)1; // g5[m] g5[l]
   const C *P[2] = {(const C *)a.propag_i, (const C *)a.propag_j};
   persistent_tiles<2>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
     const C *si = lds + g * SL, *sj = lds + (SITES_PER_WAVE + g) * SL;
@@ -37,6 +70,7 @@ __global__ __launch_bounds__(WAVE) void diquark_kernel(DiquarkArgs a)
       load_block(y, sj, j, m);
       cof_acc(acc, x, y, cmul(sparse_val<C>(a.gij, i), gl));
     }
+    if constexpr (SCALED) renormalise_output_site(acc, a.exp_i, a.exp_j, a.exp_out, first, nsites, g, q);
     __syncthreads();
     C *row = lds + g * SL + q * 9;
 #pragma unroll
@@ -46,12 +80,17 @@ __global__ __launch_bounds__(WAVE) void diquark_kernel(DiquarkArgs a)
   });
 }
 
-void launch_diquark(const DiquarkArgs &a, hipStream_t stream)
+template <bool SCALED>
+static void launch_diquark_form(const DiquarkArgs &a, hipStream_t stream)
 {
   const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(diquark_kernel, dim3((unsigned)persistent_grid(ntiles, 8)), dim3(WAVE), 0, stream, a);
+  hipLaunchKernelGGL(diquark_kernel<SCALED>, dim3((unsigned)persistent_grid(ntiles, 8)), dim3(WAVE), 0, stream, a);
 }
+
+void launch_diquark(const DiquarkArgs &a, hipStream_t stream) { launch_diquark_form<false>(a, stream); }
+
+void launch_diquark_fp32_scaled(const DiquarkArgs &a, hipStream_t stream) { launch_diquark_form<true>(a, stream); }
 
 // ---------------------------------------------------------------------------
 // Sequential source: seq_X = dC/dP_X for X in {i, j, n}
@@ -89,12 +128,7 @@ __device__ __forceinline__ void seq_term(C (&acc)[9], const C *s0, const C *s1, 
   cof_acc(acc, y, z, w);
 }
 
-// SCALED: FP32 site-scaled storage in and out.  The staged mantissas are
-// O(1) (each site's largest component in [1, 2)), so the FP32 cofactor sums
-// cannot underflow; each output site is renormalised the same way, its
-// exponent byte e_out = e_0 + e_1 + (the sum's own binary exponent), clamped
-// to [-127, 127] like the converter's, the mantissas scaled by 2^(e_0 + e_1 -
-// e_out) (exact unless the clamp pushes them below FP32's normal range).
+// SCALED: FP32 site-scaled storage in and out (renormalise_output_site).
 template <int T, int X, bool SCALED = false>
 __global__ __launch_bounds__(WAVE) void sequential_kernel(SequentialArgs a)
 {
@@ -173,24 +207,7 @@ __global__ __launch_bounds__(WAVE) void sequential_kernel(SequentialArgs a)
         seq_term<T, X>(acc, s0, s1, i, j, k, l, m, n, w);
       }
     }
-    if constexpr (SCALED) {
-      float mx = 0.0f;
-#pragma unroll
-      for (int e = 0; e < 9; ++e) mx = fmaxf(mx, fmaxf(fabsf(acc[e].x), fabsf(acc[e].y)));
-#pragma unroll
-      for (int m = 8; m >= 1; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m, 64));
-      const size_t site = first + (size_t)(g < nsites ? g : nsites - 1); // padded lanes repeat the last site
-      const int ein = (int)a.exp0[site] + (int)a.exp1[site];
-      int eout = 0;
-      if (mx > 0.0f) {
-        eout = ein + ilogbf(mx);
-        eout = eout < -SCALED_EXP_MAX ? -SCALED_EXP_MAX : (eout > SCALED_EXP_MAX ? SCALED_EXP_MAX : eout);
-      }
-      const int sh = mx > 0.0f ? ein - eout : 0;
-#pragma unroll
-      for (int e = 0; e < 9; ++e) acc[e] = make_float2(ldexpf(acc[e].x, sh), ldexpf(acc[e].y, sh));
-      if (q == 0 && g < nsites) a.exp_out[first + g] = (signed char)eout;
-    }
+    if constexpr (SCALED) renormalise_output_site(acc, a.exp0, a.exp1, a.exp_out, first, nsites, g, q);
     __syncthreads();
     C *row = lds + g * SL + q * 9;
 #pragma unroll

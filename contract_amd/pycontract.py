@@ -629,6 +629,33 @@ def baryonSequential(propag_i, propag_j, propag_n, contract_type, sequential_typ
     return props[int(sequential_type)]
 
 
+def baryonDiquarkInto(out, propag_i, propag_j, gamma_ij: int, gamma_kl: int, stream=None):
+    """contract_baryon_diquark: the diquark of propag_i and propag_j into
+    ``out``, which is returned (baryonDiquark allocates a complex128 output
+    instead).  Three complex128 tensors (FP64, as
+    baryon_diquark) or three ScaledPropagator (FP32 site-scaled storage in
+    and out)."""
+    props = (out, propag_i, propag_j)
+    scaled = [isinstance(p, ScaledPropagator) for p in props]
+    if any(scaled) and not all(scaled):
+        raise TypeError("the output and both propagators must share one storage form")
+    if all(scaled):
+        V = props[0].volume
+        for p in props:
+            if _volume(p) != V:
+                raise ValueError(f"propagators hold different site counts {[q.volume for q in props]}")
+        _same_device(props, "propagators")
+        precision = CONTRACT_FP32_SCALED
+    else:
+        V = _fp64_volume(*props)
+        precision = CONTRACT_FP64
+    rc = lib().contract_baryon_diquark(_ptr(out), _ptr(propag_i), _ptr(propag_j), precision, V, int(gamma_ij),
+                                       int(gamma_kl), 0 if stream is None else int(stream))
+    if rc != 0:
+        raise RuntimeError(f"contract_baryon_diquark failed ({rc}): {last_error()}")
+    return out
+
+
 def threePointProject(requests, insertions, momenta, outs=None, workspace=None, stream=None):
     """Three-point functions after the sequential source: for each request
     (F, Sigma, dims[, site_order[, global_dims[, origin]]]) return

@@ -45,8 +45,11 @@ KINDS = ("meson", "baryon", "diquark")
 MOM27 = [(x, y, z) for z in (-1, 0, 1) for y in (-1, 0, 1) for x in (-1, 0, 1)]
 PROTON = [(0, 1.0), (5, 1.0)]  # IK_JL_NM + IM_JL_NK on (u, d, u)
 CG5 = 5
-SITE_BYTES = {"complex128": 2304, "complex64": 1152}
-GRID_TUNING = {"complex128": (1024, 4, 2), "complex64": (1024, 8, 2)}
+# storage forms of pool entries: complex128, complex64, and the FP32 site-
+# scaled form (complex64 mantissas + one int8 exponent per site, packed from
+# FP64 as the propagator enters the pool)
+SITE_BYTES = {"complex128": 2304, "complex64": 1152, "fp32_scaled": 1153}
+GRID_TUNING = {"complex128": (1024, 4, 2), "complex64": (1024, 8, 2), "fp32_scaled": (4096, 4, 3)}
 
 
 @dataclasses.dataclass(frozen=True)
@@ -65,7 +68,10 @@ class PropKey:
 
     @property
     def seed(self) -> int:
-        return zlib.crc32(repr((self.dims, self.index, self.t0, self.alpha, self.dtype)).encode()) & 0x7FFFFFFF
+        """The generator seed: the same for every storage form of one
+        propagator, so the complex128 and the site-scaled entry of a key
+        hold the same values (up to the form's rounding)."""
+        return zlib.crc32(repr((self.dims, self.index, self.t0, self.alpha)).encode()) & 0x7FFFFFFF
 
 
 @dataclasses.dataclass(frozen=True)
@@ -225,20 +231,58 @@ class PropagatorPool:
 # GPU side
 # ---------------------------------------------------------------------------
 
+GEN_CHUNK_ENTRIES = 1 << 24  # complex entries generated per chunk (256 MiB in FP64)
+
+
+def _chunks(dims):
+    """Timeslice chunks [t0, t1) of the generator: a whole number of
+    timeslices of at most GEN_CHUNK_ENTRIES entries (one timeslice when a
+    slice is larger), fixed by the lattice alone."""
+    Lx, Ly, Lz, Lt = dims
+    per_t = Lx * Ly * Lz * 144
+    step = max(1, GEN_CHUNK_ENTRIES // per_t)
+    return [(t, min(t + step, Lt)) for t in range(0, Lt, step)]
+
+
 def device_propagator(torch, key: PropKey, device, out=None):
     """Synthetic propagator of `key` generated on the device (the stand-in for
-    loading a solved propagator into HBM), written into `out` if given."""
+    loading a solved propagator into HBM), written into `out` if given: iid
+    N(0,1) entries times exp(-alpha |t - t0|), drawn timeslice chunk by chunk
+    from (seed, chunk).  complex128 and complex64 entries are drawn in their
+    own precision; the site-scaled form is drawn in FP64 (the values of the
+    complex128 entry) and packed chunk by chunk, so the FP64 staging never
+    exceeds one chunk."""
     Lx, Ly, Lz, Lt = key.dims
-    V = Lx * Ly * Lz * Lt
-    cdt = torch.complex128 if key.dtype == "complex128" else torch.complex64
-    real = torch.float64 if key.dtype == "complex128" else torch.float32
+    S = Lx * Ly * Lz
+    V = S * Lt
+    scaled = key.dtype == "fp32_scaled"
+    cdt = torch.complex64 if key.dtype == "complex64" else torch.complex128
+    real = torch.float32 if key.dtype == "complex64" else torch.float64
     if out is None:
-        out = torch.empty((V, 4, 4, 3, 3), dtype=cdt, device=device)
-    g = torch.Generator(device=device)
-    g.manual_seed(key.seed)
-    torch.randn((V, 144, 2), dtype=real, device=device, generator=g, out=torch.view_as_real(out).view(V, 144, 2))
+        if scaled:
+            out = pc.ScaledPropagator(torch.empty(pc.scaled_fp32_bytes(V), dtype=torch.uint8, device=device), V)
+        else:
+            out = torch.empty((V, 4, 4, 3, 3), dtype=cdt, device=device)
     prof = torch.exp(-key.alpha * torch.abs(torch.arange(Lt, dtype=torch.float64, device=device) - key.t0))
-    out.view(Lt, -1, 144).mul_(prof.to(real)[:, None, None])
+    g = torch.Generator(device=device)
+    chunks = _chunks(key.dims)
+    stage = pack = None
+    if scaled:
+        most = max(t1 - t0 for t0, t1 in chunks) * S
+        stage = torch.empty((most, 4, 4, 3, 3), dtype=torch.complex128, device=device)
+        pack = torch.empty(pc.scaled_fp32_bytes(most), dtype=torch.uint8, device=device)
+        ents, exps = out.entries(), out.exponents()
+    for c, (t0, t1) in enumerate(chunks):
+        n = (t1 - t0) * S
+        dst = stage[:n] if scaled else out[t0 * S:t1 * S]
+        g.manual_seed(key.seed * 4099 + c)
+        torch.randn((n, 144, 2), dtype=real, device=device, generator=g, out=torch.view_as_real(dst).view(n, 144, 2))
+        dst.view(t1 - t0, -1, 144).mul_(prof[t0:t1].to(real)[:, None, None])
+        if scaled:
+            part = pc.toScaledFP32(dst, out=pack[:pc.scaled_fp32_bytes(n)],
+                                   stream=torch.cuda.current_stream(device).cuda_stream)
+            ents[t0 * S:t1 * S].copy_(part.entries())
+            exps[t0 * S:t1 * S].copy_(part.exponents())
     return out
 
 
@@ -294,21 +338,25 @@ class Engine:
     def _materialise(self, key, recycled=None):
         torch = self.torch
         with torch.cuda.stream(self.load_stream):
-            cdt = torch.complex128 if key.dtype == "complex128" else torch.complex64
-            reuse = self._reuse(recycled, (int(np.prod(key.dims)), 4, 4, 3, 3), cdt, self.load_stream)
+            reuse = self._reuse(recycled, key.dtype, int(np.prod(key.dims)), self.load_stream)
             t = device_propagator(torch, key, self.device, out=reuse)
             ev = torch.cuda.Event()
             ev.record(self.load_stream)
         return Entry(t, ev)
 
-    def _reuse(self, recycled, shape, dtype, stream):
-        """The memory of an evicted entry of the same byte size as a tensor of
-        `shape` and `dtype`, ordered after its last writer; None if there is
-        none."""
+    def _reuse(self, recycled, form: str, volume: int, stream):
+        """The memory of an evicted entry of the same byte size, as a
+        propagator of `volume` sites in storage form `form`, ordered after its
+        last writer; None if there is none."""
         if recycled is None or recycled.tensor is None:
             return None
         stream.wait_event(recycled.ready)
-        return recycled.tensor.reshape(-1).view(self.torch.uint8).view(dtype).view(shape)
+        t = recycled.tensor
+        raw = (t.data if isinstance(t, pc.ScaledPropagator) else t).reshape(-1).view(self.torch.uint8)
+        if form == "fp32_scaled":
+            return pc.ScaledPropagator(raw, volume)
+        cdt = self.torch.complex128 if form == "complex128" else self.torch.complex64
+        return raw.view(cdt).view(volume, 4, 4, 3, 3)
 
     def _index(self, values):
         """Small index tensor on the device, copied from pinned host memory
@@ -441,7 +489,8 @@ class Engine:
         key = r.out_key()
         with torch.cuda.stream(stream):
             pi, pj = self._inputs(r, stream)
-            out = self._reuse(self._spare.pop(r.rid, None), pi.shape, pi.dtype, stream)
+            V = int(np.prod(r.dims))
+            out = self._reuse(self._spare.pop(r.rid, None), key.dtype, V, stream)
             if out is None:
                 # pool memory is allocated on the load stream only: torch's
                 # caching allocator reuses a freed block only for allocations
@@ -450,10 +499,17 @@ class Engine:
                 # allocation costs the host tens of milliseconds).  Entries
                 # are evicted only after their last reader has finished.
                 with torch.cuda.stream(self.load_stream):
-                    out = torch.empty_like(pi)
+                    if key.dtype == "fp32_scaled":
+                        out = pc.ScaledPropagator(torch.empty(pc.scaled_fp32_bytes(V), dtype=torch.uint8,
+                                                              device=self.device), V)
+                    else:
+                        out = torch.empty_like(pi)
             (gij, gkl), = r.gammas[:1]
-            with pc.abi_stream(stream):
-                pc.baryon_diquark(out, pi, pj, int(np.prod(r.dims)), int(gij), int(gkl))
+            if key.dtype == "fp32_scaled":
+                pc.baryonDiquarkInto(out, pi, pj, int(gij), int(gkl), stream=stream.cuda_stream)
+            else:
+                with pc.abi_stream(stream):
+                    pc.baryon_diquark(out, pi, pj, V, int(gij), int(gkl))
             ev = torch.cuda.Event()
             ev.record(stream)
         self.pool.set(key, Entry(out, ev))  # its bytes were reserved at admission
@@ -738,20 +794,21 @@ def make_requests(rate: float, duration: float, seed: int, lattices=LATTICES, ca
 def sweep(rates: Sequence[float], duration: float, seed: int = 7, cap_gb: float = 200.0, streams: int = 4,
           max_inflight: int = 8, lattices=LATTICES, catalogue: int = 4, warm: bool = True,
           batch_baryons: bool = True, kind_slots: Optional[Dict[str, int]] = None,
-          min_requests: int = 60) -> Dict[str, object]:
+          min_requests: int = 60, dtype: str = "complex128") -> Dict[str, object]:
     """Serve one Poisson stream per rate on a shared engine (the pool stays warm
     between rates) and report throughput and latency per rate.  Before the
     first rate every kind of request runs once on every lattice (untimed),
-    so first launches and first allocations stay out of the windows."""
+    so first launches and first allocations stay out of the windows.  dtype:
+    the pool's storage form (complex128, complex64 or fp32_scaled)."""
     eng = Engine(cap_bytes=int(cap_gb * 1e9), streams=streams, max_inflight=max_inflight,
                  batch_baryons=batch_baryons, kind_slots=kind_slots, keep_results=False)
     if warm:
-        eng.warm_up(lattices)
+        eng.warm_up(lattices, dtype)
         eng.launches = {k: 0 for k in eng.launches}
         # load the catalogue, largest lattices last (most recently used)
         for dims in lattices:
             for i in range(catalogue):
-                k = PropKey(tuple(dims), i, t0=(7 * i) % dims[3])
+                k = PropKey(tuple(dims), i, t0=(7 * i) % dims[3], dtype=dtype)
                 if eng.pool.fits([k]):
                     eng.pool.acquire(k)
                     eng.pool.release(k)
@@ -762,7 +819,7 @@ def sweep(rates: Sequence[float], duration: float, seed: int = 7, cap_gb: float 
         # at least min_requests per rate, so that p99 is not the single
         # slowest request of a short window
         span = max(duration, min_requests / rate)
-        reqs = make_requests(rate, span, seed, lattices=lattices, catalogue=catalogue, first_rid=rid)
+        reqs = make_requests(rate, span, seed, lattices=lattices, catalogue=catalogue, first_rid=rid, dtype=dtype)
         rid += len(reqs)
         print(f"serving: rate {rate} req/s, {len(reqs)} requests", file=sys.stderr, flush=True)
         before = dict(eng.launches)
@@ -790,9 +847,12 @@ def main(argv=None):
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--max-inflight", type=int, default=8)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--storage", default="complex128", choices=sorted(SITE_BYTES),
+                    help="storage form of the pool's propagators")
     args = ap.parse_args(argv)
     rates = [float(r) for r in args.rates.split(",")]
-    print(json.dumps(sweep(rates, args.duration, args.seed, args.cap_gb, args.streams, args.max_inflight)))
+    print(json.dumps(sweep(rates, args.duration, args.seed, args.cap_gb, args.streams, args.max_inflight,
+                           dtype=args.storage)))
 
 
 if __name__ == "__main__":

@@ -126,6 +126,18 @@ int contract_baryon_twopoint_project_batch(const contract_baryon_request *reqs, 
                                            const int *momenta, int nmom, void *workspace, size_t workspace_bytes,
                                            void *stream);
 
+/* ---- diquark in FP64 or FP32 site-scaled storage ----------------------
+ * baryon_diquark (include/contract.h; the reference's lib/contract.cpp:24-28
+ * -> lib/baryon/diquark.cu:21-42) with a precision flag and an error code:
+ * CONTRACT_FP64 is the drop-in call on complex128 buffers (the same kernel,
+ * bit-identical); CONTRACT_FP32_SCALED takes both propagators in the FP32
+ * site-scaled form and writes the diquark in that form (FP32 arithmetic on
+ * the stored mantissas, each output site renormalised, its exponent byte
+ * after the V complex64 sites).  Unlike baryon_diquark, an out-of-range
+ * gamma_ij is an error, not a silent no-op. */
+int contract_baryon_diquark(void *diquark, const void *propag_i, const void *propag_j, int precision, size_t volume,
+                            int gamma_ij, int gamma_kl, void *stream);
+
 /* ---- sequential source in FP64 or FP32 site-scaled storage -------------
  * baryon_sequential_two_point (include/contract.h; the reference's
  * lib/contract.cpp:46-63) with a precision flag and an error code instead of

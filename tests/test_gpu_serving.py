@@ -26,7 +26,9 @@ def srv(cuda):
 
 def snapshot(torch, eng, r):
     if r.kind == "diquark":
-        return eng.pool.get(r.result).tensor.cpu().numpy().copy()
+        t = eng.pool.get(r.result).tensor
+        t = t.to_complex128() if hasattr(t, "to_complex128") else t
+        return t.cpu().numpy().copy()
     return r.result.cpu().numpy().copy()
 
 
@@ -143,3 +145,56 @@ def test_realtime_stream_reports_latency(srv):
     assert summary["completed"] == len(reqs)
     assert 0 < summary["p50_ms"] <= summary["p99_ms"]
     assert all(r.done >= r.arrival for r in reqs)
+
+
+def test_site_scaled_pool_matches_the_fp64_pool(srv):
+    """The same request stream served from a pool of FP32 site-scaled
+    propagators (packed from FP64 as they enter the pool; meson grids,
+    baryon batches and diquarks all in that form) and from an FP64 pool:
+    1e-6 relative per timeslice (projected results) and per propagator
+    (diquarks), the FP32 bar.  The scaled pool's cap holds five of its
+    entries, so scaled entries are evicted and their memory recycled."""
+    torch, serving, cuda = srv
+    lat = [(4, 4, 4, 8), (8, 4, 4, 4)]
+    kw = dict(lattices=lat, catalogue=4, mix=(0.4, 0.3, 0.3), max_gammas=4, max_momenta=6)
+    r64 = serving.make_requests(2000.0, 0.015, seed=9, **kw)
+    r32 = serving.make_requests(2000.0, 0.015, seed=9, dtype="fp32_scaled", **kw)
+    assert [r.kind for r in r64] == [r.kind for r in r32] and {r.kind for r in r64} == set(serving.KINDS)
+    got = {}
+    for name, reqs, cap in (("fp64", r64, 1 << 32), ("fp32_scaled", r32, 5 * 512 * 1153)):
+        eng = serving.Engine(cap_bytes=cap, streams=3, max_inflight=4)
+        out = got.setdefault(name, {})
+
+        def done(r, eng=eng, out=out):
+            t = eng.result(r)
+            t = t.to_complex128() if hasattr(t, "to_complex128") else t
+            out[r.rid] = t.cpu().numpy().copy()
+            eng.release_result(r)
+
+        eng.run(reqs, realtime=False, on_complete=done)
+        if name == "fp32_scaled":
+            assert eng.pool.evictions > 0 and eng.pool.peak <= cap
+    for a, b in zip(r64, r32):
+        assert rel_err(got["fp32_scaled"][b.rid], got["fp64"][a.rid]) < 1e-6, (a.rid, a.kind)
+    # and the FP64 pool against the oracle, as in the other tests
+    for r in r64[:6]:
+        assert rel_err(got["fp64"][r.rid], oracle_result(torch, serving, r, cuda)) < 1e-12, (r.rid, r.kind)
+
+
+def test_scaled_entry_holds_the_fp64_values(srv):
+    """A propagator generated in the site-scaled form is the complex128
+    propagator of the same key, packed (chunked generation included: the
+    chunk size is forced small so the lattice spans several chunks)."""
+    torch, serving, cuda = srv
+    from contract_amd import pycontract as pc
+    saved = serving.GEN_CHUNK_ENTRIES
+    serving.GEN_CHUNK_ENTRIES = 3 * 64 * 144  # three 4x4x4 timeslices per chunk
+    try:
+        k64 = serving.PropKey((4, 4, 4, 8), 1, t0=3)
+        k32 = serving.PropKey((4, 4, 4, 8), 1, t0=3, dtype="fp32_scaled")
+        p64 = serving.device_propagator(torch, k64, cuda)
+        p32 = serving.device_propagator(torch, k32, cuda)
+        want = pc.toScaledFP32(p64)
+        assert torch.equal(p32.data, want.data)
+    finally:
+        serving.GEN_CHUNK_ENTRIES = saved
