@@ -90,6 +90,19 @@ def test_drop_in_helpers_check_volumes_and_dtype():
         pc.baryonTwoPoint(prop(), prop(), prop(), 0, 5, 5, 8)
 
 
+def test_diquark_binding_checks_forms_and_sizes():
+    ok = pc.ScaledPropagator(torch.zeros(V * 1153, dtype=torch.uint8), V)
+    with pytest.raises(TypeError, match="one storage form"):
+        pc.baryonDiquarkInto(prop(), ok, ok, 5, 5)
+    small = pc.ScaledPropagator(torch.zeros((V - 4) * 1153, dtype=torch.uint8), V - 4)
+    with pytest.raises(ValueError, match="different site counts"):
+        pc.baryonDiquarkInto(small, ok, ok, 5, 5)
+    with pytest.raises(ValueError, match="GPU"):  # sizes pass, host tensors are refused
+        pc.baryonDiquarkInto(ok, ok, ok, 5, 5)
+    with pytest.raises(TypeError, match="complex128"):
+        pc.baryonDiquarkInto(prop(), prop(dtype=torch.complex64), prop(), 5, 5)
+
+
 def test_raw_abi_checks_tensor_spans():
     with pytest.raises(ValueError, match="propag_j holds"):
         pc.meson_two_point(torch.zeros(V, dtype=torch.complex128), prop(), prop(256), V, 5, 5)
