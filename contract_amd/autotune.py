@@ -32,7 +32,7 @@ from . import pycontract as pc
 
 DEFAULT_SITES = (256, 512, 1024, 2048, 4096)
 DEFAULT_TILES = (4, 8)
-DEFAULT_BLOCKS = (1, 2, 3)
+DEFAULT_BLOCKS = (1, 2, 3, 4)
 DEFAULT_UNITS = (1, 2)  # trace_unit: vector ALUs, matrix cores
 
 BARYON_ITEMS_PER_CU = (4, 6, 8, 12, 16)

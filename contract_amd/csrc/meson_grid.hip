@@ -69,12 +69,22 @@ __device__ __forceinline__ int scale_block(float2 (&b)[9])
 // RT row tiles of 8 momentum slots each (one pass holds up to 8 RT slots):
 // every staged site is read and traced once for all of them.  Two tiles
 // double the projection accumulators, which fit only at one wave per SIMD.
-template <typename Real, int NS, bool SCALED = false, bool KMFMA = false, int RT = 1>
+// Q4 (matrix-core traces, passes of at most 4 slots, e.g. the origin and
+// the six unit momenta): 8 of the 16 rows of a 16x16x4 projection MFMA would
+// be zero, so the projection runs as two v_mfma_f64_4x4x4f64 (four 4x4
+// blocks, a quarter of the cycles each) over rows 0-3 and 4-7.  Their lane
+// layout (measured with an in-bounds lane probe): A[b][i][k] from lane
+// 16k + 4b + i, B[b][k][j] from lane 16k + 4b + j, D[b][i][j] in lane
+// 16i + 4b + j.  With k = site g and 4b + j = column q, B is the 16x16x4
+// form's operand unchanged; A is the phase of row 4 half + (q & 3) at site
+// g; lane (g, q) ends with rows 4 half + g of column q.
+template <typename Real, int NS, bool SCALED = false, bool KMFMA = false, int RT = 1, bool Q4 = false>
 __global__ __launch_bounds__(GRID_THREADS)
 __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !KMFMA)) ? 1 : 2))) void meson_grid_kernel(
     MesonGridArgs a)
 {
   static_assert(NS % SITES_PER_WAVE == 0, "tile is a whole number of MFMA site steps");
+  static_assert(!Q4 || (KMFMA && RT == 1), "the quarter projection is a form of the one-tile matrix-core kernel");
   using C = cplx<Real>;
   using T = Tile<C>;
   constexpr int SL = T::LDS_ELEMS;               // padded site stride (complex entries)
@@ -151,6 +161,15 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
       live[rt] = pr < a.np;
       prow[rt] = live[rt] ? pr : 0;
     }
+    // Q4: this lane's A rows 4 half + (q & 3): slot 2 half + ((q >> 1) & 1), part q & 1
+    bool live4[2];
+    int prow4[2];
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+      const int pr = 2 * hf + ((q >> 1) & 1);
+      live4[hf] = pr < a.np;
+      prow4[hf] = live4[hf] ? pr : 0;
+    }
 
     // coordinates of the step's first site, advanced incrementally (uniform):
     // a slice is [z][y][xr] with xr = x (lexicographic) or x / 2 (even-odd,
@@ -171,10 +190,13 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
     // column tiles of k = w: l = 0..3 (vector traces, lane q = (i, j)) or
     // i = 0..3 (matrix-core traces, lane q = (j, l)); B = K re / K im
     dbl4 acc_re[RT][4], acc_im[RT][4];
+    double acc4_re[4][2], acc4_im[4][2]; // Q4: [i][half]
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int l = 0; l < 4; ++l) acc_re[rt][l] = acc_im[rt][l] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int l = 0; l < 4; ++l) acc4_re[l][0] = acc4_re[l][1] = acc4_im[l][0] = acc4_im[l][1] = 0.0;
 
     // the workgroup stages NS sites of both propagators per step; the next
     // step's chunks are in flight in registers while this one is computed
@@ -288,25 +310,36 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
         }
         z = z < Lz ? z : 0;
         const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
-        double av[RT];
+        auto phase = [&](int pr) {
+          const double2 ph =
+            cmul(cmul(tab[pr * (Lx + 1) + x], tab[offy + pr * (Ly + 1) + y]), tab[offz + pr * (Lz + 1) + z]);
+          return (q & 1) ? ph.y : ph.x;
+        };
+        double av[RT], av4[2];
+        if constexpr (Q4) {
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          av[rt] = 0.0;
-          if (live[rt]) {
-            const double2 ph = cmul(cmul(tab[prow[rt] * (Lx + 1) + x], tab[offy + prow[rt] * (Ly + 1) + y]),
-                                    tab[offz + prow[rt] * (Lz + 1) + z]);
-            av[rt] = (q & 1) ? ph.y : ph.x;
-          }
+          for (int hf = 0; hf < 2; ++hf) av4[hf] = live4[hf] ? phase(prow4[hf]) : 0.0;
+        } else {
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) av[rt] = live[rt] ? phase(prow[rt]) : 0.0;
         }
         if constexpr (KMFMA) {
           // rows ik = 4 i + k of this wave's k = w; B[site g][jl = q]
 #pragma unroll
           for (int i2 = 0; i2 < 4; ++i2) {
             const double2 kv = kbuf[h][g][(4 * i2 + w) * KROW + q];
+            if constexpr (Q4) {
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-              acc_re[rt][i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kv.x, acc_re[rt][i2], 0, 0, 0);
-              acc_im[rt][i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kv.y, acc_im[rt][i2], 0, 0, 0);
+              for (int hf = 0; hf < 2; ++hf) {
+                acc4_re[i2][hf] = __builtin_amdgcn_mfma_f64_4x4x4f64(av4[hf], kv.x, acc4_re[i2][hf], 0, 0, 0);
+                acc4_im[i2][hf] = __builtin_amdgcn_mfma_f64_4x4x4f64(av4[hf], kv.y, acc4_im[i2][hf], 0, 0, 0);
+              }
+            } else {
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt) {
+                acc_re[rt][i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kv.x, acc_re[rt][i2], 0, 0, 0);
+                acc_im[rt][i2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rt], kv.y, acc_im[rt][i2], 0, 0, 0);
+              }
             }
           }
           continue;
@@ -351,29 +384,43 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
     // v_mfma_f64_16x16x4f64: lane (g, q) register v holds D[g + 4v][q];
     // partial column (k l) * 16 + (i j) with k = w
     double *dst = a.partial + item * (size_t)a.partial_tiles * GRID_PARTIAL_TILE;
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
+    if constexpr (Q4) {
+      // lane (g, q) holds rows 4 half + g of column q; rows 8-15 (slots 4-7)
+      // are not read for a pass of at most 4 slots
 #pragma unroll
       for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = rt * 16 + g + 4 * v;
-          const int col = KMFMA ? (w * 4 + (q & 3)) * 16 + t * 4 + (q >> 2) : (w * 4 + t) * 16 + q;
-          dst[(row * 2 + 0) * GRID_COLS + col] = acc_re[rt][t][v];
-          dst[(row * 2 + 1) * GRID_COLS + col] = acc_im[rt][t][v];
+        for (int hf = 0; hf < 2; ++hf) {
+          const int row = 4 * hf + g;
+          const int col = (w * 4 + (q & 3)) * 16 + t * 4 + (q >> 2);
+          dst[(row * 2 + 0) * GRID_COLS + col] = acc4_re[t][hf];
+          dst[(row * 2 + 1) * GRID_COLS + col] = acc4_im[t][hf];
         }
+    } else {
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = rt * 16 + g + 4 * v;
+            const int col = KMFMA ? (w * 4 + (q & 3)) * 16 + t * 4 + (q >> 2) : (w * 4 + t) * 16 + q;
+            dst[(row * 2 + 0) * GRID_COLS + col] = acc_re[rt][t][v];
+            dst[(row * 2 + 1) * GRID_COLS + col] = acc_im[rt][t][v];
+          }
+    }
   }
 }
 
-template <typename Real, bool SCALED, bool KMFMA, int RT>
+template <typename Real, bool SCALED, bool KMFMA, int RT, bool Q4 = false>
 static void launch_grid_rt(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
 {
   if (tile_sites == 8)
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 8, SCALED, KMFMA, RT>), dim3((unsigned)blocks), dim3(GRID_THREADS),
-                       shmem, stream, a);
+    hipLaunchKernelGGL((meson_grid_kernel<Real, 8, SCALED, KMFMA, RT, Q4>), dim3((unsigned)blocks),
+                       dim3(GRID_THREADS), shmem, stream, a);
   else
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 4, SCALED, KMFMA, RT>), dim3((unsigned)blocks), dim3(GRID_THREADS),
-                       shmem, stream, a);
+    hipLaunchKernelGGL((meson_grid_kernel<Real, 4, SCALED, KMFMA, RT, Q4>), dim3((unsigned)blocks),
+                       dim3(GRID_THREADS), shmem, stream, a);
 }
 
 template <typename Real, bool SCALED, bool KMFMA>
@@ -381,6 +428,8 @@ static void launch_grid_t(const MesonGridArgs &a, int tile_sites, size_t blocks,
 {
   if (a.np > GRID_MOM_PER_TILE)
     launch_grid_rt<Real, SCALED, KMFMA, 2>(a, tile_sites, blocks, shmem, stream);
+  else if (KMFMA && a.np <= GRID_MOM_PER_TILE / 2)
+    launch_grid_rt<Real, SCALED, KMFMA, 1, KMFMA>(a, tile_sites, blocks, shmem, stream);
   else
     launch_grid_rt<Real, SCALED, KMFMA, 1>(a, tile_sites, blocks, shmem, stream);
 }
