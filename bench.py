@@ -716,6 +716,16 @@ def serving_leg(torch, args):
     catalogue of 4 FP64 propagators per lattice held in an LRU pool capped at
     200 GB; one stream of `serving_duration` seconds per offered rate."""
     from contract_amd import serving
+
+    def memory(what):
+        free, total = torch.cuda.mem_get_info()
+        print(f"bench: {what}: allocated {torch.cuda.memory_allocated() / 1e9:.1f} GB, reserved "
+              f"{torch.cuda.memory_reserved() / 1e9:.1f} GB, device free {free / 1e9:.1f} of {total / 1e9:.1f} GB",
+              file=sys.stderr, flush=True)
+
+    gc.collect()
+    torch.cuda.empty_cache()
+    memory("before the serving sweep")
     rates = [float(r) for r in args.serving_rates.split(",")]
     res = serving.sweep(rates, args.serving_duration, seed=7, cap_gb=200.0, streams=4, max_inflight=8)
     res["workload"] = ("Poisson arrivals, 50% meson / 30% baryon / 20% diquark, lattices 16^3x32, 24^3x64, "
@@ -725,6 +735,7 @@ def serving_leg(torch, args):
     if args.serving_scaled_rates:
         # the same request sequence against a pool of FP32 site-scaled
         # propagators (packed from FP64 as they enter the pool)
+        memory("before the site-scaled serving sweep")
         progress("serving sweep, site-scaled FP32 pool")
         rs = [float(r) for r in args.serving_scaled_rates.split(",")]
         res["fp32_scaled"] = serving.sweep(rs, args.serving_duration, seed=7, cap_gb=200.0, streams=4,

@@ -570,10 +570,13 @@ class Engine:
         rid = -1
         for dims in lattices:
             keys = [PropKey(tuple(dims), i, t0=(7 * i) % dims[3], dtype=dtype) for i in range(2)]
-            reqs.append(Request(rid, "meson", tuple(dims), (keys[0], keys[1]), [(5, 5)], MOM27))
-            reqs.append(Request(rid - 1, "baryon", tuple(dims), (keys[0], keys[1], keys[0]), [0, 8], MOM27))
-            reqs.append(Request(rid - 2, "diquark", tuple(dims), (keys[0], keys[1]), [(5, 5)], []))
-            rid -= 3
+            # grid passes of <= 4, 5-8 and 9-16 momentum slots run three kernel forms
+            for moms in (MOM27[:2], MOM27[:6], MOM27):
+                reqs.append(Request(rid, "meson", tuple(dims), (keys[0], keys[1]), [(5, 5)], moms))
+                rid -= 1
+            reqs.append(Request(rid, "baryon", tuple(dims), (keys[0], keys[1], keys[0]), [0, 8], MOM27))
+            reqs.append(Request(rid - 1, "diquark", tuple(dims), (keys[0], keys[1]), [(5, 5)], []))
+            rid -= 2
         keep = self.keep_results
         self.keep_results = False
         try:
