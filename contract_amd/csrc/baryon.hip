@@ -185,6 +185,32 @@ __device__ __forceinline__ C baryon_tail(const C (&D)[9], const C *__restrict__ 
   }
 }
 
+// IK_JL_NM tail whose LDS scratch is P_n's own staging area (for kernels that
+// have no other area free): every lane reads its P_n block into registers
+// first, then the group's site area holds the X1 sum, formed in the same
+// order as baryon_tail<0> so the two give bit-identical results.  Any other
+// tail that reads P_n runs before this one.
+template <typename C>
+__device__ __forceinline__ C baryon_tail0_in_place(const C (&D)[9], C *__restrict__ sn, int lane, const KlLane<C> &kl)
+{
+  const int q = lane & 15;
+  C a[9];
+  load_block(a, sn, q & 3, q >> 2);
+  const C w = kl.template wz<0>();
+  const int slot = q & 3;
+  wave_sync(); // the group's P_n reads are done
+  if (w.x != 0 || w.y != 0)
+#pragma unroll
+    for (int e = 0; e < 9; ++e) sn[slot * 9 + e] = cmul(w, D[e]);
+  wave_sync();
+  if (q < 9) sn[36 + q] = cadd(cadd(sn[q], sn[9 + q]), cadd(sn[18 + q], sn[27 + q]));
+  wave_sync();
+  C x1[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) x1[e] = sn[36 + e];
+  return dot9(a, x1);
+}
+
 template <typename C>
 __device__ __forceinline__ C baryon_tail_rt(int type, const C (&D)[9], const C *sn, C *scr, int lane,
                                             const KlLane<C> &kl)
@@ -305,13 +331,18 @@ enum { TERMS_RUNTIME = 0, TERMS_NUCLEON = 1 };
 // The staged form (the FP64 nucleon at one momentum tile) holds two
 // propagators in LDS at a time and skips the register prefetch of the whole
 // next tile, which buys a second wave per SIMD (18.5 KB of LDS per wave,
-// eight waves per CU, half the register file).  Each wave loads P_i (and
-// P_n) at the top of the step while the other wave on the SIMD computes;
-// P_n waits in registers while the diquark is formed, then takes P_i's
-// consumed staging area.  The one-wave FP64 forms, bound by their own step
-// latency, hide the loads behind the step instead.  (The FP32 forms already
-// run two waves per SIMD with the full prefetch; staged at three waves they
-// measured 8-17% slower.)
+// eight waves per CU, half the register file).  With three distinct
+// propagators every load of the next tile is in flight across at least half
+// a step: P_j is register-prefetched before the diquark; P_i goes HBM -> LDS
+// directly (dma_tile) into an area of its own once the diquark has consumed
+// it; P_n is stored into P_j's consumed area mid-step and its next tile is
+// loaded into the freed registers right after.  Steps order LDS with
+// wave_sync, since a __syncthreads() would wait for the P_i load.  With
+// P_n = P_i (shared) the one propagator stays in LDS through the tails and
+// is loaded at the top of the step.  The one-wave FP64 forms, bound by their
+// own step latency, hide the loads behind the step instead.  (The FP32 forms
+// already run two waves per SIMD with the full prefetch; staged at three
+// waves they measured 8-17% slower.)
 template <typename Real, int NT, int TERMS> constexpr bool project_staged()
 {
   return sizeof(Real) == 8 && TERMS == TERMS_NUCLEON && NT == 1;
@@ -337,9 +368,14 @@ baryon_project_kernel(
   constexpr bool PERSIST = PREFETCH && project_persistent<Real, NT>();
   constexpr bool PF_J = STAGED; // staged form: P_j a step ahead, P_i (and P_n) at the top of the step
   constexpr int NP = SHARE ? 2 : 3;   // distinct propagators read per site
-  constexpr int NL = STAGED ? 2 : NP; // staging areas
   constexpr bool MID_N = STAGED && !SHARE; // P_n staged after the diquark
+  // Staged three-propagator form: P_i goes HBM -> LDS directly into an area
+  // of its own, issued once the diquark has consumed the previous tile (half
+  // a step ahead); P_j and then P_n share the register-staged area.
+  constexpr bool DMA_I = MID_N;
+  constexpr int NL = DMA_I ? 1 : (STAGED ? 2 : NP); // register-staged areas
   __shared__ C lds[NL][SITES_PER_WAVE * SL];
+  __shared__ C lds_i[DMA_I ? SITES_PER_WAVE * SITE_ELEMS : 1]; // unpadded: dma_tile's layout
   extern __shared__ double2 tab[]; // Gx + Gy + Gz phase entries, sized at launch
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
@@ -352,6 +388,14 @@ baryon_project_kernel(
   load_phase_tab(tab, args.req[r], lane);
   const KlLane<C> kl(a.gkl, a.gklT, q);
 
+  // one-wave workgroups: with P_i's HBM -> LDS load in flight across the
+  // step, a __syncthreads() would wait for it (see wave_sync)
+  auto step_sync = [] {
+    if constexpr (DMA_I)
+      wave_sync();
+    else
+      __syncthreads();
+  };
   auto tile_sites = [](size_t s, size_t end) {
     const size_t left = end - s;
     return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
@@ -374,6 +418,10 @@ baryon_project_kernel(
     if constexpr (SCALED) ex_buf = site_exp(pa, first, n);
   } else if constexpr (PF_J) {
     buf[1].load((const C *)args.req[r].propag_j, first, tile_sites(first, last), lane);
+    if constexpr (DMA_I) {
+      dma_tile(lds_i, (const C *)args.req[r].propag_i, first, tile_sites(first, last), lane);
+      buf[2].load((const C *)args.req[r].propag_n, first, tile_sites(first, last), lane);
+    }
   }
 
   for (;;) {
@@ -400,16 +448,20 @@ baryon_project_kernel(
       for (int u = 0; u < 4; ++u) acc[tt][u] = dbl4{0.0, 0.0, 0.0, 0.0};
 
     for (size_t s = first; s < last; s += SITES_PER_WAVE) {
+      const int nhere = tile_sites(s, last);
       if constexpr (STAGED) {
-        const int n = tile_sites(s, last);
-        buf[0].load(Pi, s, n, lane);
-        if constexpr (!PF_J) buf[1].load(Pj, s, n, lane);
-        if constexpr (!SHARE) buf[2].load(Pn, s, n, lane);
-        if constexpr (SCALED) ex_buf = site_exp(pa, s, n);
+        if constexpr (!DMA_I) buf[0].load(Pi, s, nhere, lane);
+        if constexpr (!PF_J) buf[1].load(Pj, s, nhere, lane);
+        if constexpr (!SHARE && !DMA_I) buf[2].load(Pn, s, nhere, lane);
+        if constexpr (SCALED) ex_buf = site_exp(pa, s, nhere);
       }
+      if constexpr (DMA_I) {
+        buf[1].store(lds[0], lane);
+      } else {
 #pragma unroll
-      for (int p = 0; p < NL; ++p) buf[p].store(lds[p], lane);
-      __syncthreads();
+        for (int p = 0; p < NL; ++p) buf[p].store(lds[p], lane);
+      }
+      step_sync();
       int ex = ex_buf;
       const size_t next = s + SITES_PER_WAVE;
       const bool here = next < last; // else the next item's first tile (uniform)
@@ -424,8 +476,9 @@ baryon_project_kernel(
         if constexpr (SCALED) ex_buf = site_exp(here ? pa : args.req[r_next], st, n);
       }
 
-      C *si = lds[0] + g * SL, *sj = lds[1] + g * SL, *sn = lds[NL == 2 ? 0 : 2] + g * SL;
-      C *scr = NL == 2 ? sj : si; // consumed once D is formed
+      C *si = DMA_I ? lds_i + g * SITE_ELEMS : lds[0] + g * SL;
+      C *sj = lds[DMA_I ? 0 : 1] + g * SL, *sn = lds[NL == 3 ? 2 : 0] + g * SL;
+      C *scr = NL == 2 ? sj : si; // consumed once D is formed (unused with DMA_I)
       if constexpr (sizeof(Real) == 4 && !SCALED) {
         // P_i and P_j only: D of the rescaled blocks is O(1), so the tails'
         // products with P_n are as large as P_n's own entries and cannot
@@ -440,16 +493,32 @@ baryon_project_kernel(
         diquark_tensor_static<GIJ>(D, si, sj, q);
       else
         diquark_tensor(D, si, sj, q, a.gij);
-      if constexpr (MID_N) { // P_n into P_i's consumed area
-        __syncthreads();
+      // DMA_I: the phase table (dynamic LDS, which the compiler cannot tell
+      // apart from P_i's area) is read before P_i's next load is issued, or
+      // its reads would wait for that load
+      double ph_re[NT], ph_im[NT];
+      if constexpr (DMA_I) project_phases<NT>(ph_re, ph_im, args.np, pa, tab, walk);
+      if constexpr (MID_N) { // P_n into P_j's consumed area; the next P_i into its own
+        step_sync();
+        dma_tile(lds_i, Pi, next, here ? tile_sites(next, last) : 0, lane); // an empty range past the item
         buf[2].store(lds[0], lane);
-        __syncthreads();
+        if constexpr (DMA_I) // the next P_n, a whole step ahead
+          buf[2].load(Pn, next, here ? tile_sites(next, last) : 0, lane);
+        step_sync();
       }
       double2 R = make_double2(0.0, 0.0);
       if constexpr (TERMS == TERMS_NUCLEON) { // types {IK_JL_NM, IM_JL_NK}, in that order
-        const C r0 = baryon_tail<0, C>(D, sn, scr, lane, kl);
+        // With DMA_I no LDS area is free for the IK_JL_NM sum: it runs last,
+        // in P_n's area (the terms still enter R in type order).
+        C r0, r5;
+        if constexpr (DMA_I) {
+          r5 = baryon_tail<5, C>(D, sn, scr, lane, kl);
+          r0 = baryon_tail0_in_place<C>(D, sn, lane, kl);
+        } else {
+          r0 = baryon_tail<0, C>(D, sn, scr, lane, kl);
+          r5 = baryon_tail<5, C>(D, sn, scr, lane, kl);
+        }
         R = cfma(R, make_double2(a.w_re[0], a.w_im[0]), make_double2((double)r0.x, (double)r0.y));
-        const C r5 = baryon_tail<5, C>(D, sn, scr, lane, kl);
         R = cfma(R, make_double2(a.w_re[1], a.w_im[1]), make_double2((double)r5.x, (double)r5.y));
       } else {
         for (int t = 0; t < a.nterm; ++t) {
@@ -458,9 +527,14 @@ baryon_project_kernel(
         }
       }
       if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
-      project_accumulate<NT>(acc, args.np, pa, tab, R, walk); // padded lanes carry R = 0
+      if constexpr (DMA_I) { // a ragged tile's padded sites read stale P_i slots
+        R = g < nhere ? R : make_double2(0.0, 0.0);
+        project_mfma<NT>(acc, ph_re, ph_im, R);
+      } else {
+        project_accumulate<NT>(acc, args.np, pa, tab, R, walk); // padded lanes carry R = 0
+      }
       walk.advance(pa);
-      __syncthreads();
+      step_sync();
     }
     project_store<NT>(acc, args.np, pa, item - pa.item0, lane);
     if (!more) break;

@@ -44,9 +44,13 @@ template <typename C> __device__ __forceinline__ C cmake(decltype(C::x) re, decl
 template <typename C> __device__ __forceinline__ C czero() { return cmake<C>(0, 0); }
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmake<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmake<C>(a.x - b.x, a.y - b.y); }
+// The fused form is spelled out: left to the compiler, `a.x * b.x - a.y * b.y`
+// may contract into either FMA, and the choice can differ between kernels
+// that compute the same value (the bit-identity of kernel variants relies on
+// one rounding).
 template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
-  return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return cmake<C>(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 // acc + a * b
 template <typename C> __device__ __forceinline__ C cfma(C acc, C a, C b)
@@ -244,6 +248,41 @@ template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
     }
   }
 };
+
+// HBM -> LDS without registers (gfx950 `buffer_load_dwordx4 ... lds`): lane l
+// of instruction it writes chunk e = it * 64 + l to the 16-byte LDS slot e, so
+// the tile lands unpadded (site stride SITE16 chunks).  Range-limited like
+// Tile::load; the slots of chunks past `nsites` must not be relied on (callers
+// zero what those sites produce).  Completion is tracked by vmcnt: the
+// compiler waits for it before the first LDS read of the destination array.
+template <typename C>
+__device__ __forceinline__ void dma_tile(C *__restrict__ lds, const C *__restrict__ src, size_t first_site,
+                                         int nsites, int lane)
+{
+  using T = Tile<C>;
+  static_assert(T::CHUNKS % WAVE == 0, "whole instructions per tile");
+  typedef __attribute__((address_space(3))) void lds_void;
+  const __amdgpu_buffer_rsrc_t rsrc = tile_rsrc(src + first_site * SITE_ELEMS, (unsigned)(nsites * T::SITE16 * 16));
+  uint4 *l16 = reinterpret_cast<uint4 *>(lds);
+#pragma unroll
+  for (int it = 0; it < T::PER_LANE; ++it)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void *)(l16 + it * WAVE), 16, lane * 16, it * WAVE * 16, 0,
+                                             BUFFER_AUX_NT); // lane offset in a VGPR, instruction offset in an SGPR
+}
+
+// Orders the LDS traffic between the lanes of a one-wave workgroup (a wave's
+// LDS instructions execute in issue order, so finishing the earlier ones is
+// enough).  __syncthreads() is a workgroup release fence, which also waits for
+// every HBM -> LDS load in flight (vmcnt(0)); a kernel that keeps dma_tile
+// loads in flight across such a point uses this instead, and the compiler
+// still waits for a dma_tile before the first read of its destination.
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0): the wave's earlier LDS reads and writes are done
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <typename C> constexpr int site_lds() { return Tile<C>::LDS_ELEMS; }
 constexpr int SITE_LDS = Tile<double2>::LDS_ELEMS; // FP64 padded site stride

@@ -139,11 +139,12 @@ template <int NT> struct PhaseWalk {
 // Momentum slot k (one per +-k pair) keeps C = sum cos(k.x) R and
 // S = -sum sin(k.x) R in four accumulators (C re, C im, S re, S im); the finish
 // forms P(+-k) = C +- iS.
+// This lane's phase rows (cos, -sin) at its site of the step; rows past np
+// are zero.
 template <int NT>
-__device__ __forceinline__ void project_accumulate(dbl4 (&acc)[NT][4], int np, const ProjectReq &pa,
-                                                   const double2 *tab, double2 R, const PhaseWalk<NT> &walk)
+__device__ __forceinline__ void project_phases(double (&pr)[NT], double (&pi)[NT], int np, const ProjectReq &pa,
+                                               const double2 *tab, const PhaseWalk<NT> &walk)
 {
-  double pr[NT], pi[NT];
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
     const double2 ph = walk.phase(pa, tab, tt); // (cos, -sin)
@@ -151,6 +152,27 @@ __device__ __forceinline__ void project_accumulate(dbl4 (&acc)[NT][4], int np, c
     pr[tt] = live ? ph.x : 0.0;
     pi[tt] = live ? ph.y : 0.0;
   }
+}
+
+template <int NT>
+__device__ __forceinline__ void project_mfma(dbl4 (&acc)[NT][4], const double (&pr)[NT], const double (&pi)[NT],
+                                             double2 R)
+{
+#pragma unroll
+  for (int tt = 0; tt < NT; ++tt) {
+    acc[tt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.x, acc[tt][0], 0, 0, 0);
+    acc[tt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.y, acc[tt][1], 0, 0, 0);
+    acc[tt][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[tt], R.x, acc[tt][2], 0, 0, 0);
+    acc[tt][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi[tt], R.y, acc[tt][3], 0, 0, 0);
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void project_accumulate(dbl4 (&acc)[NT][4], int np, const ProjectReq &pa,
+                                                   const double2 *tab, double2 R, const PhaseWalk<NT> &walk)
+{
+  double pr[NT], pi[NT];
+  project_phases<NT>(pr, pi, np, pa, tab, walk);
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
     acc[tt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[tt], R.x, acc[tt][0], 0, 0, 0);
