@@ -96,10 +96,12 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
   // each wave forms K of its sites w, w + 4, ... of the tile, one barrier
   // hands them over, then every wave projects its columns of all of them;
   // the next tile's writes follow the barrier at the top of its step.  Rows
-  // padded to 17 entries and sites to 16 * 17 + 1, so neither the MFMA-layout
-  // writes (lane groups on rows g + 4v) nor the B-operand reads (lane groups
-  // on sites) of a wave fall on one LDS bank
-  constexpr int KROW = 17, KSITE = 16 * KROW + 1;
+  // of 16 entries and sites of 256 (unpadded): a ds_read/write_b128 of a wave
+  // is conflict-free when each 16-lane group touches 16 distinct banks and
+  // lane q of every group the same bank (group offsets a multiple of 256 B;
+  // measured with an in-bounds LDS probe), which both the MFMA-layout writes
+  // (groups on rows g + 4v) and the B-operand reads (groups on sites) meet
+  constexpr int KROW = 16, KSITE = 16 * KROW;
   constexpr int NH = NS / SITES_PER_WAVE; // four-site groups per tile
   __shared__ double2 kbuf[KMFMA ? NH : 1][KMFMA ? SITES_PER_WAVE : 1][KSITE];
   extern __shared__ double2 tab[];
