@@ -631,7 +631,7 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle, fi
             "mfma_TFLOPs": 16384.0 * sites / (kern * 1e-3) / 1e12,
         }
         if name == "fp64":
-            res[name]["traffic"], res[name]["traffic_source"] = measured_traffic("meson_grid_kernel<double,8>")
+            res[name]["traffic"], res[name]["traffic_source"] = measured_traffic("meson_grid_kernel<double,4,q4>")
         outs[name] = [o.cpu().numpy() for o in out]
         if name == "fp64" and check_oracle:
             import oracle
