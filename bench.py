@@ -1071,16 +1071,18 @@ def north_star(kern64, achieved, kern32, kern32s, err32, err32s, err32c, err32sc
     if threept:
         ns["threepoint"] = threept.get("summary")
     if serving:
-        ns["serving"] = {r: [_r(v["throughput_rps"]), _r(v["p50_ms"], 3), _r(v["p99_ms"], 3),
-                             _r(v["meson_batch_mean"], 3), _r(v["baryon_batch_mean"], 3)]
-                         for r, v in serving["rates"].items()}
-        ns["serving_columns"] = "rate: [served req/s, p50 ms, p99 ms, meson batch mean, baryon batch mean]"
+        def rows(rates):
+            return {r: [_r(v.get("arrival_rps") or float(r)), _r(v["throughput_rps"]), _r(v["p50_ms"], 3),
+                        _r(v["p99_ms"], 3), _r(v["meson_batch_mean"], 3), _r(v["baryon_batch_mean"], 3)]
+                    for r, v in rates.items()}
+
+        ns["serving"] = rows(serving["rates"])
+        ns["serving_columns"] = ("nominal rate: [realised arrivals req/s, served req/s, p50 ms, p99 ms, "
+                                 "meson batch mean, baryon batch mean]")
         ns["serving_saturation"] = serving.get("saturation")
         sc = serving.get("fp32_scaled")
         if sc:
-            ns["serving_fp32_scaled"] = {r: [_r(v["throughput_rps"]), _r(v["p50_ms"], 3), _r(v["p99_ms"], 3),
-                                             _r(v["meson_batch_mean"], 3), _r(v["baryon_batch_mean"], 3)]
-                                         for r, v in sc["rates"].items()}
+            ns["serving_fp32_scaled"] = rows(sc["rates"])
             ns["serving_fp32_scaled_saturation"] = sc.get("saturation")
     return ns
 

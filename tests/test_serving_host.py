@@ -204,3 +204,33 @@ def test_saturation_summary():
     s = saturation(per)
     assert s["max_sustained_offered_rps"] == 100.0 and s["saturated"]
     assert s["peak_throughput_rps"] == 240.0 and s["at_offered_rps"] == 300.0
+
+
+def test_saturation_uses_realised_arrivals_and_p99():
+    from contract_amd.serving import saturation
+    # 300 req/s nominal, 320 realised, 272 served with a growing queue: not
+    # sustained although 272 >= 0.9 x 300
+    per = {"100.0": {"throughput_rps": 113.7, "arrival_rps": 114.0, "p99_ms": 54.0},
+           "200.0": {"throughput_rps": 215.7, "arrival_rps": 216.5, "p99_ms": 175.0},
+           "300.0": {"throughput_rps": 272.2, "arrival_rps": 320.0, "p99_ms": 1520.0}}
+    s = saturation(per)
+    assert s["max_sustained_offered_rps"] == 200.0 and s["saturated"]
+    # the throughput keeps up but p99 is past the bound: not sustained either
+    per["200.0"]["p99_ms"] = 600.0
+    assert saturation(per)["max_sustained_offered_rps"] == 100.0
+
+
+def test_summary_records_realised_arrival_rate():
+    from contract_amd import serving
+
+    class R:
+        def __init__(self, t):
+            self.arrival, self.done, self.latency, self.kind = t, t + 0.01, 0.01, "meson"
+
+    class P:
+        cap = peak = hits = misses = evictions = 0
+
+    reqs = [R(0.1 * i) for i in range(11)]  # 10 gaps over 1 s
+    out = serving.summarise(reqs, 1.1, P(), {"meson": [1], "baryon": []}, {})
+    assert abs(out["arrival_rps"] - 10.0) < 1e-9
+    assert serving.summarise([R(0.0), R(0.0)], 0.1, P(), {"meson": [], "baryon": []}, {})["arrival_rps"] is None
