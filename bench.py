@@ -618,7 +618,8 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle, fi
         gbs = bps * sites / (kern * 1e-3) / 1e9
         res[name] = {
             "tuning": {"sites_per_item": best[0], "tile_sites": best[1], "blocks_per_cu": best[2],
-                       "trace_unit": {0: "library default", 1: "vector ALUs", 2: "matrix cores"}[
+                       "trace_unit": {0: "library default", 1: "vector ALUs", 2: "matrix cores",
+                                      3: "matrix cores, wave-local K"}[
                            best[3] if len(best) > 3 else 0]},
             "autotune_s": t_tune,
             "autotune_ms": {"/".join(str(x) for x in k): round(v, 3) for k, v in sorted(table.items())},

@@ -33,7 +33,8 @@ from . import pycontract as pc
 DEFAULT_SITES = (256, 512, 1024, 2048, 4096)
 DEFAULT_TILES = (4, 8)
 DEFAULT_BLOCKS = (1, 2, 3, 4)
-DEFAULT_UNITS = (1, 2)  # trace_unit: vector ALUs, matrix cores
+DEFAULT_UNITS = (1, 2, 3)  # trace_unit: vector ALUs, matrix cores (shared tile), matrix cores (wave-local K)
+WAVE_LOCAL_BLOCKS = (4,)   # trace_unit 3: one-wave workgroups, one per SIMD (tile fixed at 4 sites)
 
 BARYON_ITEMS_PER_CU = (4, 6, 8, 12, 16)
 BARYON_MIN_SITES = (64, 128, 256, 512)
@@ -190,7 +191,13 @@ def clear_cache(file: bool = False) -> None:
 def candidates(sites: Sequence[int] = DEFAULT_SITES, tiles: Sequence[int] = DEFAULT_TILES,
                blocks: Sequence[int] = DEFAULT_BLOCKS, units: Sequence[int] = DEFAULT_UNITS
                ) -> List[Tuple[int, int, int, int]]:
-    return [(s, t, b, u) for s, t, b, u in itertools.product(sites, tiles, blocks, units)]
+    """(sites_per_item, tile_sites, blocks_per_cu, trace_unit) plans.  The
+    wave-local form (unit 3) stages 4-site tiles in one-wave workgroups, one
+    per SIMD: only its sites per item vary."""
+    out = [(s, t, b, u) for s, t, b, u in itertools.product(sites, tiles, blocks, [u for u in units if u != 3])]
+    if 3 in units:
+        out += [(s, 4, b, 3) for s in sites for b in WAVE_LOCAL_BLOCKS]
+    return out
 
 
 def _units_allowed(requests) -> Sequence[int]:

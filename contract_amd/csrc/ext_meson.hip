@@ -39,12 +39,12 @@ GridTuning resolve_tuning(const contract_meson_grid_tuning *t, int precision)
 
 int check_tuning(const GridTuning &g, int precision)
 {
-  if (g.unit != 1 && g.unit != 2) {
-    set_last_error("trace_unit = %d is not 0, 1 or 2", g.unit);
+  if (g.unit < 1 || g.unit > 3) {
+    set_last_error("trace_unit = %d is not 0, 1, 2 or 3", g.unit);
     return CONTRACT_ERR_ARG;
   }
-  if (g.unit == 2 && precision == CONTRACT_FP32) {
-    set_last_error("trace_unit = 2 (matrix cores) needs CONTRACT_FP64 or CONTRACT_FP32_SCALED storage");
+  if (g.unit >= 2 && precision == CONTRACT_FP32) {
+    set_last_error("trace_unit = %d (matrix cores) needs CONTRACT_FP64 or CONTRACT_FP32_SCALED storage", g.unit);
     return CONTRACT_ERR_ARG;
   }
   if (g.tile != 4 && g.tile != 8) {
@@ -247,7 +247,7 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
       launch_meson_grid(a,
                         precision == CONTRACT_FP32_SCALED ? GRID_FP32_SCALED
                                                           : (precision == CONTRACT_FP32 ? GRID_FP32 : GRID_FP64),
-                        g.unit == 2, g.tile, grid, (size_t)tab * a.np, stream);
+                        g.unit, g.tile, grid, (size_t)tab * a.np, stream);
       CONTRACT_EXT_HIP(hipGetLastError());
       f.np = a.np;
       launch_meson_grid_finish(f, blocks, stream);

@@ -414,6 +414,272 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
   }
 }
 
+// The two 32-bit halves of a double, for the lane-row swaps below.
+__device__ __forceinline__ void swap_rows32(double &a, double &b)
+{
+  const unsigned long long ua = __builtin_bit_cast(unsigned long long, a);
+  const unsigned long long ub = __builtin_bit_cast(unsigned long long, b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  a = __builtin_bit_cast(double, (unsigned long long)hi[0] << 32 | lo[0]);
+  b = __builtin_bit_cast(double, (unsigned long long)hi[1] << 32 | lo[1]);
+}
+__device__ __forceinline__ void swap_rows16(double &a, double &b)
+{
+  const unsigned long long ua = __builtin_bit_cast(unsigned long long, a);
+  const unsigned long long ub = __builtin_bit_cast(unsigned long long, b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  a = __builtin_bit_cast(double, (unsigned long long)hi[0] << 32 | lo[0]);
+  b = __builtin_bit_cast(double, (unsigned long long)hi[1] << 32 | lo[1]);
+}
+__device__ __forceinline__ void swap_rows32(float &a, float &b)
+{
+  const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                  false, false);
+  a = __builtin_bit_cast(float, (unsigned)r[0]);
+  b = __builtin_bit_cast(float, (unsigned)r[1]);
+}
+__device__ __forceinline__ void swap_rows16(float &a, float &b)
+{
+  const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                  false, false);
+  a = __builtin_bit_cast(float, (unsigned)r[0]);
+  b = __builtin_bit_cast(float, (unsigned)r[1]);
+}
+
+// Transpose of 4 registers x 4 lane rows (16 lanes each): on return row s of
+// r[g] holds what row g of r[s] held.  v_permlane32_swap(a, b) exchanges the
+// upper half of a with the lower half of b, v_permlane16_swap(a, b) the odd
+// rows of a with the even rows of b (lane layouts measured with an in-bounds
+// probe), so swaps of (0,2), (1,3) and then (0,1), (2,3) transpose.
+template <typename R> __device__ __forceinline__ void transpose_rows4(R &r0, R &r1, R &r2, R &r3)
+{
+  swap_rows32(r0, r2);
+  swap_rows32(r1, r3);
+  swap_rows16(r0, r1);
+  swap_rows16(r2, r3);
+}
+
+// Matrix-core traces with the K hand-over in registers (trace_unit 3), for
+// passes of at most 4 momentum slots: a workgroup is one wave, which stages
+// a 4-site tile of both propagators (register prefetch of the next), forms K
+// of all four sites (the 16x16x4 trace MFMAs above, lane (g, q) register v
+// holding K_s[row(g, v)][q]), turns the four sites' registers into the
+// projection's B operand (lane row k = site k) with lane-row transposes
+// instead of an LDS round trip and a workgroup barrier, and accumulates all
+// 256 columns x (K re, K im) x 8 rows itself with v_mfma_f64_4x4x4f64 (the
+// quarter-tile projection of meson_grid_kernel; A = the phase of row
+// 4 half + (q & 3) at the site of lane row g).  The 64 accumulators per lane
+// live in the matrix-core registers.  Waves are independent: no LDS is shared
+// and no barrier is taken.  Partials land in the same layout as
+// meson_grid_kernel's, so the finish kernel is shared.
+template <typename Real, bool SCALED>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(1, 1))) void meson_grid_wave_kernel(
+    MesonGridArgs a)
+{
+  using C = cplx<Real>;
+  using T = Tile<C>;
+  using KAcc = typename std::conditional<sizeof(Real) == 8, dbl4, flt4>::type;
+  constexpr int SL = T::LDS_ELEMS;
+  __shared__ C lds[2][SITES_PER_WAVE * SL]; // [propagator]
+  extern __shared__ double2 tab[];
+  const int lane = threadIdx.x;
+  const int g = lane >> 4, q = lane & 15;
+  // Q4: this lane's A rows 4 half + (q & 3): slot 2 half + ((q >> 1) & 1), part q & 1
+  bool live4[2];
+  int prow4[2];
+#pragma unroll
+  for (int hf = 0; hf < 2; ++hf) {
+    const int pr = 2 * hf + ((q >> 1) & 1);
+    live4[hf] = pr < a.np;
+    prow4[hf] = live4[hf] ? pr : 0;
+  }
+  int rtab = -1; // request whose phase tables are in LDS
+
+  for (size_t item = blockIdx.x; item < a.total_items; item += gridDim.x) {
+    int r = 0;
+    while (r + 1 < a.nreq && item >= a.req[r + 1].item0) ++r;
+    const MesonGridReq &R = a.req[r];
+    const size_t local = item - R.item0;
+    const size_t slice = local / R.chunks_per_slice;
+    const size_t c = local - slice * R.chunks_per_slice;
+    const size_t s0 = slice * R.slice_sites;
+    const size_t first = s0 + c * (size_t)a.chunk;
+    size_t last = first + a.chunk;
+    last = last < s0 + R.slice_sites ? last : s0 + R.slice_sites;
+
+    const int Lx = R.dims[0], Ly = R.dims[1], Lz = R.dims[2];
+    const int offy = a.np * (Lx + 1), offz = a.np * (Lx + Ly + 2);
+    if (r != rtab) { // the tables depend on the request only
+      wave_sync();   // the previous item's readers of tab are done
+      const int ntab = a.np * (Lx + Ly + Lz + 3);
+      for (int e = lane; e < ntab; e += WAVE) {
+        const int mu = e < offy ? 0 : (e < offz ? 1 : 2);
+        const int L = R.dims[mu], G = R.gdims[mu];
+        const int rel = e - (mu == 0 ? 0 : (mu == 1 ? offy : offz));
+        const int p = rel / (L + 1), k = rel - p * (L + 1);
+        const int m = nonneg_mod(a.mom[p][mu], G);
+        double sn, cs;
+        sincospi(2.0 * (double)((m * ((k + R.origin[mu]) % G)) % G) / (double)G, &sn, &cs);
+        tab[e] = make_double2(cs, -sn);
+      }
+      rtab = r;
+    }
+
+    // coordinates of the step's first site, advanced incrementally (as in
+    // meson_grid_kernel)
+    const int lt = R.dims[3];
+    const int par = (int)(slice / lt), tsl = (int)(slice - (size_t)par * lt);
+    const int Lr = R.order ? Lx >> 1 : Lx;
+    int x0, y0, z0;
+    {
+      const unsigned r0 = (unsigned)(c * (size_t)a.chunk);
+      x0 = (int)(r0 % (unsigned)Lr);
+      const unsigned rest = r0 / (unsigned)Lr;
+      y0 = (int)(rest % (unsigned)Ly);
+      z0 = (int)(rest / (unsigned)Ly);
+    }
+    const int pbase = par + tsl;
+
+    double acc[16][2][2]; // [row ik of K][K re, K im][half]
+#pragma unroll
+    for (int ik = 0; ik < 16; ++ik)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) acc[ik][u][0] = acc[ik][u][1] = 0.0;
+
+    const C *P0 = static_cast<const C *>(R.propag_i), *P1 = static_cast<const C *>(R.propag_j);
+    auto tile_n = [&](size_t st) { // sites of the tile at st (past the item's end read as zero)
+      const size_t left = last - st;
+      return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
+    };
+    T bi, bj;
+    bi.load(P0, first, tile_n(first), lane);
+    bj.load(P1, first, tile_n(first), lane);
+    for (size_t st = first; st < last; st += SITES_PER_WAVE) {
+      wave_sync(); // the previous step's readers of the tile are done
+      bi.store(lds[0], lane);
+      bj.store(lds[1], lane);
+      const size_t next = st + SITES_PER_WAVE;
+      const int nnext = next < last ? tile_n(next) : 0; // an empty range past the item
+      bi.load(P0, next < last ? next : st, nnext, lane);
+      bj.load(P1, next < last ? next : st, nnext, lane);
+      wave_sync();
+
+      // traces of the four sites
+      KAcc Kr[SITES_PER_WAVE], Ki[SITES_PER_WAVE];
+#pragma unroll
+      for (int sk = 0; sk < SITES_PER_WAVE; ++sk) {
+        const C *xs = lds[0] + sk * SL, *ys = lds[1] + sk * SL;
+        KAcc kr = KAcc{0, 0, 0, 0}, ki = KAcc{0, 0, 0, 0}, t3 = KAcc{0, 0, 0, 0};
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+          const int cc = 4 * ch + g;
+          const C xv = xs[q * 9 + cc], yv = ys[q * 9 + cc];
+          if constexpr (sizeof(Real) == 8) {
+            kr = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.x, yv.x, kr, 0, 0, 0);
+            ki = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.y, yv.y, ki, 0, 0, 0);
+            t3 = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.x + xv.y, yv.x - yv.y, t3, 0, 0, 0);
+          } else {
+            kr = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, yv.x, kr, 0, 0, 0);
+            kr = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, yv.y, kr, 0, 0, 0);
+            ki = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, yv.x, ki, 0, 0, 0);
+            ki = __builtin_amdgcn_mfma_f32_16x16x4f32(-xv.x, yv.y, ki, 0, 0, 0);
+          }
+        }
+        if constexpr (sizeof(Real) == 8) {
+          const KAcc t1 = kr, t2 = ki;
+          kr = t1 + t2;
+          ki = t3 - t1 + t2;
+        }
+        const C yv = ys[q * 9 + 8]; // colour pair 8 on the vector ALUs
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = sizeof(Real) == 8 ? g + 4 * v : 4 * g + v;
+          const C t = cfma_conj(cmake<C>(kr[v], ki[v]), xs[row * 9 + 8], yv);
+          kr[v] = t.x;
+          ki[v] = t.y;
+        }
+        Kr[sk] = kr;
+        Ki[sk] = ki;
+      }
+      // lane row g of site register sk holds rows row(g, v) of K_sk; after the
+      // transpose, lane row k of register g holds row row(g, v) of K_k
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        Real r0 = Kr[0][v], r1 = Kr[1][v], r2 = Kr[2][v], r3 = Kr[3][v];
+        transpose_rows4(r0, r1, r2, r3);
+        Kr[0][v] = r0, Kr[1][v] = r1, Kr[2][v] = r2, Kr[3][v] = r3;
+        Real i0 = Ki[0][v], i1 = Ki[1][v], i2 = Ki[2][v], i3 = Ki[3][v];
+        transpose_rows4(i0, i1, i2, i3);
+        Ki[0][v] = i0, Ki[1][v] = i1, Ki[2][v] = i2, Ki[3][v] = i3;
+      }
+      int ex = 0;
+      if constexpr (SCALED) { // this lane row's site (a site past the item reads the last one; its K is 0)
+        const size_t site = st + (size_t)g < last ? st + (size_t)g : last - 1;
+        ex = (int)R.exp_i[site] + (int)R.exp_j[site];
+      }
+
+      // this lane row's site, its coordinates and phases
+      int xr = x0 + g, y = y0, z = z0;
+      while (xr >= Lr) {
+        xr -= Lr;
+        if (++y == Ly) {
+          y = 0;
+          ++z;
+        }
+      }
+      z = z < Lz ? z : 0;
+      const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
+      double av4[2];
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        const int pr = prow4[hf];
+        const double2 ph =
+          cmul(cmul(tab[pr * (Lx + 1) + x], tab[offy + pr * (Ly + 1) + y]), tab[offz + pr * (Lz + 1) + z]);
+        av4[hf] = live4[hf] ? ((q & 1) ? ph.y : ph.x) : 0.0;
+      }
+#pragma unroll
+      for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ik = sizeof(Real) == 8 ? gg + 4 * v : 4 * gg + v;
+          double re = (double)Kr[gg][v], im = (double)Ki[gg][v];
+          if constexpr (SCALED) {
+            re = ldexp(re, ex);
+            im = ldexp(im, ex);
+          }
+#pragma unroll
+          for (int hf = 0; hf < 2; ++hf) {
+            acc[ik][0][hf] = __builtin_amdgcn_mfma_f64_4x4x4f64(av4[hf], re, acc[ik][0][hf], 0, 0, 0);
+            acc[ik][1][hf] = __builtin_amdgcn_mfma_f64_4x4x4f64(av4[hf], im, acc[ik][1][hf], 0, 0, 0);
+          }
+        }
+      x0 += SITES_PER_WAVE;
+      while (x0 >= Lr) {
+        x0 -= Lr;
+        if (++y0 == Ly) {
+          y0 = 0;
+          ++z0;
+        }
+      }
+    }
+    // D[b][i][j] of v_mfma_f64_4x4x4f64 in lane 16 i + 4 b + j: lane (g, q)
+    // holds row 4 half + g of column q = jl of row ik; partial column
+    // (k l) * 16 + (i j) for ik = 4 i + k, jl = 4 j + l
+    double *dst = a.partial + item * (size_t)a.partial_tiles * GRID_PARTIAL_TILE;
+#pragma unroll
+    for (int ik = 0; ik < 16; ++ik)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        const int row = 4 * hf + g;
+        const int col = ((ik & 3) * 4 + (q & 3)) * 16 + (ik >> 2) * 4 + (q >> 2);
+        dst[(row * 2 + 0) * GRID_COLS + col] = acc[ik][0][hf];
+        dst[(row * 2 + 1) * GRID_COLS + col] = acc[ik][1][hf];
+      }
+  }
+}
+
 template <typename Real, bool SCALED, bool KMFMA, int RT, bool Q4 = false>
 static void launch_grid_rt(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
 {
@@ -436,10 +702,19 @@ static void launch_grid_t(const MesonGridArgs &a, int tile_sites, size_t blocks,
     launch_grid_rt<Real, SCALED, KMFMA, 1>(a, tile_sites, blocks, shmem, stream);
 }
 
-void launch_meson_grid(const MesonGridArgs &a, int storage, bool mfma_traces, int tile_sites, size_t blocks,
+void launch_meson_grid(const MesonGridArgs &a, int storage, int unit, int tile_sites, size_t blocks,
                        size_t tab_entries, hipStream_t stream)
 {
   const size_t shmem = tab_entries * sizeof(double2);
+  const bool mfma_traces = unit >= 2;
+  if (unit == 3 && a.np <= GRID_MOM_PER_TILE / 2) { // wave-local K; other passes take the shared-tile form
+    if (storage == GRID_FP32_SCALED)
+      hipLaunchKernelGGL((meson_grid_wave_kernel<float, true>), dim3((unsigned)blocks), dim3(WAVE), shmem, stream, a);
+    else
+      hipLaunchKernelGGL((meson_grid_wave_kernel<double, false>), dim3((unsigned)blocks), dim3(WAVE), shmem, stream,
+                         a);
+    return;
+  }
   if (storage == GRID_FP32_SCALED) {
     if (mfma_traces)
       launch_grid_t<float, true, true>(a, tile_sites, blocks, shmem, stream);

@@ -61,10 +61,14 @@ struct MesonGridFinish {
   SparseSpin g5t[16]; // (g5 G(g))^T: row i -> (j, (g5 G(g))[j][i])
 };
 
-// tile_sites: sites staged per step (4 or 8); blocks: persistent grid size;
-// tab_entries: np x max(Lx + Ly + Lz + 3), the per-momentum phase tables in LDS.
+// unit: 1 vector-ALU traces, 2 matrix-core traces (four waves share a staged
+// tile), 3 matrix-core traces with a wave-local K (one-wave workgroups;
+// passes of at most 4 momentum slots, others run unit 2); tile_sites: sites
+// staged per step (4 or 8, units 1 and 2); blocks: persistent grid size
+// (workgroups); tab_entries: np x max(Lx + Ly + Lz + 3), the per-momentum
+// phase tables in LDS.
 enum GridStorage { GRID_FP64 = 0, GRID_FP32 = 1, GRID_FP32_SCALED = 2 };
-void launch_meson_grid(const MesonGridArgs &a, int storage, bool mfma_traces, int tile_sites, size_t blocks,
+void launch_meson_grid(const MesonGridArgs &a, int storage, int unit, int tile_sites, size_t blocks,
                        size_t tab_entries, hipStream_t stream);
 void launch_meson_grid_finish(const MesonGridFinish &f, int nblocks, hipStream_t stream);
 

@@ -721,7 +721,8 @@ def _fill_geometry(r, req):
 
 class MesonGridTuning(ctypes.Structure):
     """contract_meson_grid_tuning; zero fields take the library default.
-    trace_unit: 1 vector ALUs, 2 matrix cores (FP64 / site-scaled FP32)."""
+    trace_unit: 1 vector ALUs, 2 matrix cores (FP64 / site-scaled FP32), 3 matrix cores
+    with one wave per 4-site tile holding all of K (passes of at most 4 momentum slots)."""
     _fields_ = [("sites_per_item", ctypes.c_int), ("tile_sites", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
                 ("trace_unit", ctypes.c_int)]
 

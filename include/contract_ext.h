@@ -233,7 +233,11 @@ typedef struct {
   int tile_sites;     /* sites staged per step: 4 or 8 */
   int blocks_per_cu;  /* persistent workgroups per compute unit: 1..8 */
   int trace_unit;     /* per-site traces on 1: the vector ALUs, 2: the matrix
-                         cores (FP64 and FP32 site-scaled storage only);
+                         cores (FP64 and FP32 site-scaled storage only),
+                         3: the matrix cores with one wave per 4-site tile
+                         holding all of K (same storage; passes of at most 4
+                         momentum slots, other passes run as 2; tile_sites
+                         ignored, blocks_per_cu counts one-wave workgroups);
                          0: the library's choice */
 } contract_meson_grid_tuning;
 

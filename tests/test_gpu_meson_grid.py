@@ -289,3 +289,50 @@ def test_many_slots_several_two_tile_passes(dev):
     assert grid_error(got, oracle_grid(host, moms)) < 1e-12
     with pytest.raises(RuntimeError, match="workspace"):
         pc.mesonGridProject([req], moms, tuning=tuning, workspace=ws[:n7])
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("waves", [1, 4, 8])
+def test_wave_local_traces_match_oracle(dev, order, waves):
+    """trace_unit 3 (one wave per 4-site tile, K moved to the projection's
+    operand layout by lane-row transposes): an uneven batch with ragged items
+    (27-site slices) in both site orders, FP64 at 1e-12 and site-scaled FP32
+    at 1e-6, and 1-7 momenta (passes of at most 4 slots)."""
+    torch, pc, _ = dev
+    reqs, hosts = uneven_batch(dev)
+    # 27-site slices (per parity for even-odd order): every item ends ragged
+    r, h = make_request(dev, (6, 3, 3, 4) if order else (3, 3, 3, 4), 78, 1, order)
+    reqs.append(r)
+    hosts.append(h)
+    for moms in (MOM7, MOM7[:1], MOM7[:3]):
+        outs = pc.mesonGridProject(reqs, moms, tuning=(12, 4, waves, 3))
+        for out, hh in zip(outs, hosts):
+            assert grid_error(out.cpu().numpy(), oracle_grid(hh, moms)) < 1e-12, moms
+    f32, h32 = uneven_batch(dev, np.complex64)
+    outs = pc.mesonGridProject(scaled_batch(dev, f32), MOM7, tuning=(16, 4, waves, 3))
+    for out, hh in zip(outs, h32):
+        assert timeslice_error(out.cpu().numpy(), oracle_grid(hh, MOM7)) < 1e-6
+
+
+def test_wave_local_traces_other_passes(dev):
+    """More than 4 momentum slots: trace_unit 3 runs those passes in the
+    shared-tile form and the result still matches the oracle."""
+    torch, pc, _ = dev
+    r, h = make_request(dev, (4, 2, 4, 4), 21, 0)
+    got = pc.mesonGridProject([r], MOM27, tuning=(16, 4, 4, 3))[0].cpu().numpy()
+    assert grid_error(got, oracle_grid(h, MOM27)) < 1e-12
+
+
+def test_wave_local_equals_shared_tile_form_to_rounding(dev):
+    """The same batch and work items through trace_unit 2 and 3: the same
+    products in a different accumulation order, equal to rounding; and a
+    request is bit-identical alone and in the batch under trace_unit 3."""
+    torch, pc, _ = dev
+    reqs, _ = uneven_batch(dev)
+    two = [o.cpu().numpy() for o in pc.mesonGridProject(reqs, MOM7, tuning=(16, 4, 2, 2))]
+    three = [o.cpu().numpy() for o in pc.mesonGridProject(reqs, MOM7, tuning=(16, 4, 4, 3))]
+    for a, b in zip(two, three):
+        assert grid_error(b, a) < 1e-13
+    for r, m in zip(reqs, three):
+        alone = pc.mesonGridProject([r], MOM7, tuning=(16, 4, 4, 3))[0].cpu().numpy()
+        assert np.array_equal(alone, m)
