@@ -34,7 +34,7 @@ DEFAULT_SITES = (256, 512, 1024, 2048, 4096)
 DEFAULT_TILES = (4, 8)
 DEFAULT_BLOCKS = (1, 2, 3, 4)
 DEFAULT_UNITS = (1, 2, 3)  # trace_unit: vector ALUs, matrix cores (shared tile), matrix cores (wave-local K)
-WAVE_LOCAL_BLOCKS = (4,)   # trace_unit 3: one-wave workgroups, one per SIMD (tile fixed at 4 sites)
+WAVE_LOCAL_BLOCKS = (4, 8)  # trace_unit 3: one-wave workgroups, one or two per SIMD (tile fixed at 4 sites)
 
 BARYON_ITEMS_PER_CU = (4, 6, 8, 12, 16)
 BARYON_MIN_SITES = (64, 128, 256, 512)

@@ -375,7 +375,7 @@ baryon_project_kernel(
   constexpr bool DMA_I = MID_N;
   constexpr int NL = DMA_I ? 1 : (STAGED ? 2 : NP); // register-staged areas
   __shared__ C lds[NL][SITES_PER_WAVE * SL];
-  __shared__ C lds_i[DMA_I ? SITES_PER_WAVE * SITE_ELEMS : 1]; // unpadded: dma_tile's layout
+  __shared__ C lds_i[DMA_I ? Tile<C>::DMA_ELEMS : 1]; // unpadded: dma_tile's layout
   extern __shared__ double2 tab[]; // Gx + Gy + Gz phase entries, sized at launch
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;

@@ -224,6 +224,7 @@ template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
   static constexpr int LDS_ELEMS = LDS16 * 16 / (int)sizeof(C);       // per site, in complex
   static constexpr int CHUNKS = NSITE * SITE16;                       // per tile
   static constexpr int PER_LANE = (CHUNKS + WAVE - 1) / WAVE;         // 4 sites: 9 / 5; 2 sites: 5 / 3
+  static constexpr int DMA_ELEMS = PER_LANE * WAVE * 16 / (int)sizeof(C); // a dma_tile destination, in complex
   uint4 v[PER_LANE];
 
   // Chunks past nsites (ragged tile) and lanes past the tile read as zero;
@@ -251,16 +252,18 @@ template <typename C, int NSITE = SITES_PER_WAVE> struct Tile {
 
 // HBM -> LDS without registers (gfx950 `buffer_load_dwordx4 ... lds`): lane l
 // of instruction it writes chunk e = it * 64 + l to the 16-byte LDS slot e, so
-// the tile lands unpadded (site stride SITE16 chunks).  Range-limited like
-// Tile::load; the slots of chunks past `nsites` must not be relied on (callers
-// zero what those sites produce).  Completion is tracked by vmcnt: the
-// compiler waits for it before the first LDS read of the destination array.
+// the tile lands unpadded (site stride SITE16 chunks).  The destination holds
+// Tile::DMA_ELEMS (PER_LANE * 64 slots: whole instructions, an FP32 tile's
+// last one is half past the tile).  Range-limited like Tile::load (the
+// instruction offset goes in the scalar offset, which the range check
+// includes): chunks past `nsites` are not read and land as zeros (both
+// measured with an in-bounds probe).  Completion is tracked by vmcnt; see
+// dma_wait.
 template <typename C>
 __device__ __forceinline__ void dma_tile(C *__restrict__ lds, const C *__restrict__ src, size_t first_site,
                                          int nsites, int lane)
 {
   using T = Tile<C>;
-  static_assert(T::CHUNKS % WAVE == 0, "whole instructions per tile");
   typedef __attribute__((address_space(3))) void lds_void;
   const __amdgpu_buffer_rsrc_t rsrc = tile_rsrc(src + first_site * SITE_ELEMS, (unsigned)(nsites * T::SITE16 * 16));
   uint4 *l16 = reinterpret_cast<uint4 *>(lds);
@@ -280,6 +283,20 @@ __device__ __forceinline__ void wave_sync()
 {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0): the wave's earlier LDS reads and writes are done
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Waits for every HBM -> LDS load (dma_tile) of the wave, and orders the
+// LDS reads that follow after it.  The compiler normally inserts this wait
+// before the first read of a dma_tile destination, but not in every loop
+// shape (the meson grid's wave-local kernel read its tiles with no wait), so
+// a kernel whose only loads in flight at that point are its tile loads
+// states it.  Out-of-range lanes of a dma_tile write zeros (measured).
+__device__ __forceinline__ void dma_wait()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0)
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }

@@ -474,15 +474,53 @@ template <typename R> __device__ __forceinline__ void transpose_rows4(R &r0, R &
 // live in the matrix-core registers.  Waves are independent: no LDS is shared
 // and no barrier is taken.  Partials land in the same layout as
 // meson_grid_kernel's, so the finish kernel is shared.
-template <typename Real, bool SCALED>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(1, 1))) void meson_grid_wave_kernel(
-    MesonGridArgs a)
+//
+// DMA (site-scaled FP32): the tiles move HBM -> LDS directly (dma_tile), the
+// next one issued once this step's traces have read the current one; without
+// the 40 registers of the prefetch the kernel fits two waves per SIMD.  The
+// step's phases and site exponents are read before that load is issued (a
+// later read of the dynamic-LDS table or a later global load would wait for
+// it), and a ragged tile's padded sites get K = 0.
+// The site terms of a wave-kernel step (inside its loop): the site exponent
+// sum EX of lane row g's site (site-scaled storage) and the phases AV[half]
+// of this lane's A rows at that site.  A macro so that each kernel form can
+// place it where its register pressure allows.
+#define GRID_SITE_TERMS(EX, AV)                                                                             \
+  do {                                                                                                      \
+    if constexpr (SCALED) { /* a site past the item reads the last one; its K is 0 */                      \
+      const size_t site_ = st + (size_t)g < last ? st + (size_t)g : last - 1;                               \
+      EX = (int)R.exp_i[site_] + (int)R.exp_j[site_];                                                       \
+    }                                                                                                       \
+    int xr_ = x0 + g, y_ = y0, z_ = z0;                                                                     \
+    while (xr_ >= Lr) {                                                                                     \
+      xr_ -= Lr;                                                                                            \
+      if (++y_ == Ly) {                                                                                     \
+        y_ = 0;                                                                                             \
+        ++z_;                                                                                               \
+      }                                                                                                     \
+    }                                                                                                       \
+    z_ = z_ < Lz ? z_ : 0;                                                                                  \
+    const int x_ = R.order ? 2 * xr_ + ((pbase + y_ + z_) & 1) : xr_;                                       \
+    for (int hf_ = 0; hf_ < 2; ++hf_) {                                                                     \
+      const int pr_ = prow4[hf_];                                                                           \
+      const double2 ph_ = cmul(cmul(tab[pr_ * (Lx + 1) + x_], tab[offy + pr_ * (Ly + 1) + y_]),           \
+                               tab[offz + pr_ * (Lz + 1) + z_]);                                            \
+      AV[hf_] = live4[hf_] ? ((q & 1) ? ph_.y : ph_.x) : 0.0;                                               \
+    }                                                                                                       \
+  } while (0)
+
+template <typename Real, bool SCALED, bool DMA>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(DMA ? 2 : 1, DMA ? 2 : 1))) void
+meson_grid_wave_kernel(MesonGridArgs a)
 {
   using C = cplx<Real>;
   using T = Tile<C>;
   using KAcc = typename std::conditional<sizeof(Real) == 8, dbl4, flt4>::type;
-  constexpr int SL = T::LDS_ELEMS;
-  __shared__ C lds[2][SITES_PER_WAVE * SL]; // [propagator]
+  constexpr int SL = DMA ? SITE_ELEMS : T::LDS_ELEMS; // site stride of the staged tiles
+  __shared__ C lds[DMA ? 1 : 2][DMA ? 1 : SITES_PER_WAVE * SL]; // [propagator], register-staged
+  __shared__ C dma_i[DMA ? T::DMA_ELEMS : 1], dma_j[DMA ? T::DMA_ELEMS : 1];
+  C *const tile_i = DMA ? dma_i : lds[0];
+  C *const tile_j = DMA ? dma_j : lds[DMA ? 0 : 1];
   extern __shared__ double2 tab[];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
@@ -554,23 +592,33 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(1, 1))) vo
       return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
     };
     T bi, bj;
-    bi.load(P0, first, tile_n(first), lane);
-    bj.load(P1, first, tile_n(first), lane);
+    if constexpr (DMA) {
+      dma_wait(); // the previous item's last (empty) tile load has landed
+      dma_tile(dma_i, P0, first, tile_n(first), lane);
+      dma_tile(dma_j, P1, first, tile_n(first), lane);
+    } else {
+      bi.load(P0, first, tile_n(first), lane);
+      bj.load(P1, first, tile_n(first), lane);
+    }
     for (size_t st = first; st < last; st += SITES_PER_WAVE) {
-      wave_sync(); // the previous step's readers of the tile are done
-      bi.store(lds[0], lane);
-      bj.store(lds[1], lane);
       const size_t next = st + SITES_PER_WAVE;
       const int nnext = next < last ? tile_n(next) : 0; // an empty range past the item
-      bi.load(P0, next < last ? next : st, nnext, lane);
-      bj.load(P1, next < last ? next : st, nnext, lane);
-      wave_sync();
+      if constexpr (DMA) {
+        dma_wait(); // this tile's loads, the only ones in flight here
+      } else {
+        wave_sync(); // the previous step's readers of the tile are done
+        bi.store(lds[0], lane);
+        bj.store(lds[1], lane);
+        bi.load(P0, next < last ? next : st, nnext, lane);
+        bj.load(P1, next < last ? next : st, nnext, lane);
+        wave_sync();
+      }
 
       // traces of the four sites
       KAcc Kr[SITES_PER_WAVE], Ki[SITES_PER_WAVE];
 #pragma unroll
       for (int sk = 0; sk < SITES_PER_WAVE; ++sk) {
-        const C *xs = lds[0] + sk * SL, *ys = lds[1] + sk * SL;
+        const C *xs = tile_i + sk * SL, *ys = tile_j + sk * SL;
         KAcc kr = KAcc{0, 0, 0, 0}, ki = KAcc{0, 0, 0, 0}, t3 = KAcc{0, 0, 0, 0};
 #pragma unroll
         for (int ch = 0; ch < 2; ++ch) {
@@ -600,8 +648,27 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(1, 1))) vo
           kr[v] = t.x;
           ki[v] = t.y;
         }
+        if constexpr (DMA) { // padded sites of a ragged tile (zero-filled by dma_tile; kept exact here)
+          if (st + (size_t)sk >= last) kr = ki = KAcc{0, 0, 0, 0};
+        }
         Kr[sk] = kr;
         Ki[sk] = ki;
+      }
+      // this lane row's site exponents, coordinates and phases: with DMA read
+      // before the next tile's load is issued (a dynamic-LDS or global read
+      // after it would wait for it), otherwise after the transposes (fewer
+      // registers live across them)
+      int ex = 0;
+      double av4[2];
+      if constexpr (DMA) {
+        GRID_SITE_TERMS(ex, av4);
+      }
+      if constexpr (DMA) { // the tile has been read: the next one into its place
+        wave_sync();
+        // past the item an empty range (its zeros land before the next
+        // item's first tile is loaded: that load waits for them)
+        dma_tile(dma_i, P0, next < last ? next : st, nnext, lane);
+        dma_tile(dma_j, P1, next < last ? next : st, nnext, lane);
       }
       // lane row g of site register sk holds rows row(g, v) of K_sk; after the
       // transpose, lane row k of register g holds row row(g, v) of K_k
@@ -614,30 +681,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(1, 1))) vo
         transpose_rows4(i0, i1, i2, i3);
         Ki[0][v] = i0, Ki[1][v] = i1, Ki[2][v] = i2, Ki[3][v] = i3;
       }
-      int ex = 0;
-      if constexpr (SCALED) { // this lane row's site (a site past the item reads the last one; its K is 0)
-        const size_t site = st + (size_t)g < last ? st + (size_t)g : last - 1;
-        ex = (int)R.exp_i[site] + (int)R.exp_j[site];
-      }
-
-      // this lane row's site, its coordinates and phases
-      int xr = x0 + g, y = y0, z = z0;
-      while (xr >= Lr) {
-        xr -= Lr;
-        if (++y == Ly) {
-          y = 0;
-          ++z;
-        }
-      }
-      z = z < Lz ? z : 0;
-      const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
-      double av4[2];
-#pragma unroll
-      for (int hf = 0; hf < 2; ++hf) {
-        const int pr = prow4[hf];
-        const double2 ph =
-          cmul(cmul(tab[pr * (Lx + 1) + x], tab[offy + pr * (Ly + 1) + y]), tab[offz + pr * (Lz + 1) + z]);
-        av4[hf] = live4[hf] ? ((q & 1) ? ph.y : ph.x) : 0.0;
+      if constexpr (!DMA) {
+        GRID_SITE_TERMS(ex, av4);
       }
 #pragma unroll
       for (int gg = 0; gg < 4; ++gg)
@@ -680,6 +725,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(1, 1))) vo
   }
 }
 
+#undef GRID_SITE_TERMS
+
 template <typename Real, bool SCALED, bool KMFMA, int RT, bool Q4 = false>
 static void launch_grid_rt(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
 {
@@ -709,10 +756,11 @@ void launch_meson_grid(const MesonGridArgs &a, int storage, int unit, int tile_s
   const bool mfma_traces = unit >= 2;
   if (unit == 3 && a.np <= GRID_MOM_PER_TILE / 2) { // wave-local K; other passes take the shared-tile form
     if (storage == GRID_FP32_SCALED)
-      hipLaunchKernelGGL((meson_grid_wave_kernel<float, true>), dim3((unsigned)blocks), dim3(WAVE), shmem, stream, a);
+      hipLaunchKernelGGL((meson_grid_wave_kernel<float, true, true>), dim3((unsigned)blocks), dim3(WAVE), shmem,
+                         stream, a);
     else
-      hipLaunchKernelGGL((meson_grid_wave_kernel<double, false>), dim3((unsigned)blocks), dim3(WAVE), shmem, stream,
-                         a);
+      hipLaunchKernelGGL((meson_grid_wave_kernel<double, false, false>), dim3((unsigned)blocks), dim3(WAVE), shmem,
+                         stream, a);
     return;
   }
   if (storage == GRID_FP32_SCALED) {

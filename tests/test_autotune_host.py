@@ -66,7 +66,7 @@ def test_candidates_hold_the_defaults():
     assert autotune.BARYON_DEFAULT in autotune.baryon_candidates(items=(4,), sites=(64,))
     c = autotune.candidates()
     assert (1024, 4, 2, 2) in c and (1024, 4, 4, 3) in c
-    assert len(c) == len(set(c)) == 5 * 2 * 4 * 2 + 5
+    assert len(c) == len(set(c)) == 5 * 2 * 4 * 2 + 5 * 2
     assert all(u != 3 for *_, u in autotune.candidates(units=(1, 2)))
 
 
