@@ -24,17 +24,34 @@ struct GridTuning {
   int chunk, tile, bpc, unit; // unit: 1 vector-ALU traces, 2 matrix-core traces
 };
 
+// Library defaults, each the fastest plan for its storage on a batch of 24
+// lattices from 16^3x32 to 32^3x64 at 7 momenta: 1024 sites per item; FP64
+// the shared-tile matrix-core form, tile 4; plain FP32 the vector-ALU form,
+// tile 8; site-scaled FP32 the wave-local form.  A zero blocks_per_cu takes
+// the default of the form each pass runs (pass_blocks_per_cu).
 GridTuning resolve_tuning(const contract_meson_grid_tuning *t, int precision)
 {
-  GridTuning g{1024, 4, 2, 0};
+  GridTuning g{1024, 0, 0, 0};
   if (t) {
-    if (t->tile_sites) g.tile = t->tile_sites;
+    g.tile = t->tile_sites;
     if (t->sites_per_item) g.chunk = t->sites_per_item;
-    if (t->blocks_per_cu) g.bpc = t->blocks_per_cu;
+    g.bpc = t->blocks_per_cu;
     g.unit = t->trace_unit;
   }
-  if (g.unit == 0) g.unit = precision == CONTRACT_FP32 ? 1 : 2;
+  if (g.unit == 0) g.unit = precision == CONTRACT_FP32 ? 1 : (precision == CONTRACT_FP32_SCALED ? 3 : 2);
+  if (g.tile == 0) g.tile = precision == CONTRACT_FP32 && g.unit == 1 ? 8 : 4;
   return g;
+}
+
+// Persistent workgroups per CU of a pass: the one-wave workgroups of the
+// wave-local form fill one SIMD each (two for site-scaled storage, whose
+// form fits two waves per SIMD); three four-wave workgroups for the
+// shared-tile matrix-core form, two for the vector-ALU form.
+int pass_blocks_per_cu(const GridTuning &g, bool wave_pass, int precision)
+{
+  if (g.bpc) return g.bpc;
+  if (wave_pass) return precision == CONTRACT_FP32_SCALED ? 8 : 4;
+  return g.unit >= 2 ? 3 : 2;
 }
 
 int check_tuning(const GridTuning &g, int precision)
@@ -55,8 +72,8 @@ int check_tuning(const GridTuning &g, int precision)
     set_last_error("sites_per_item = %d out of range 1..%d", g.chunk, 1 << 24);
     return CONTRACT_ERR_ARG;
   }
-  if (g.bpc < 1 || g.bpc > 8) {
-    set_last_error("blocks_per_cu = %d out of range 1..8", g.bpc);
+  if (g.bpc < 0 || g.bpc > 8) {
+    set_last_error("blocks_per_cu = %d out of range 1..8 (0: the default)", g.bpc);
     return CONTRACT_ERR_ARG;
   }
   return CONTRACT_OK;
@@ -235,9 +252,10 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
       tab = std::max(tab, q.dims[0] + q.dims[1] + q.dims[2] + 3);
     }
     a.total_items = (unsigned)items;
-    const size_t grid = std::min(items, (size_t)device_cu_count() * g.bpc);
     for (size_t p0 = 0; p0 < slots.size(); p0 += GRID_MOM_PER_PASS) {
       a.np = (int)std::min(slots.size() - p0, (size_t)GRID_MOM_PER_PASS);
+      const size_t grid =
+          std::min(items, (size_t)device_cu_count() * pass_blocks_per_cu(g, grid_wave_pass(g.unit, a.np), precision));
       for (int p = 0; p < a.np; ++p) {
         const MomSlot &m = slots[p0 + p];
         for (int mu = 0; mu < 3; ++mu) a.mom[p][mu] = momenta[3 * m.rep + mu];

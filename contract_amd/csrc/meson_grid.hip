@@ -754,7 +754,7 @@ void launch_meson_grid(const MesonGridArgs &a, int storage, int unit, int tile_s
 {
   const size_t shmem = tab_entries * sizeof(double2);
   const bool mfma_traces = unit >= 2;
-  if (unit == 3 && a.np <= GRID_MOM_PER_TILE / 2) { // wave-local K; other passes take the shared-tile form
+  if (grid_wave_pass(unit, a.np)) { // wave-local K; other passes take the shared-tile form
     if (storage == GRID_FP32_SCALED)
       hipLaunchKernelGGL((meson_grid_wave_kernel<float, true, true>), dim3((unsigned)blocks), dim3(WAVE), shmem,
                          stream, a);

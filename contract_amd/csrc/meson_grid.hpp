@@ -68,6 +68,9 @@ struct MesonGridFinish {
 // (workgroups); tab_entries: np x max(Lx + Ly + Lz + 3), the per-momentum
 // phase tables in LDS.
 enum GridStorage { GRID_FP64 = 0, GRID_FP32 = 1, GRID_FP32_SCALED = 2 };
+// whether launch_meson_grid runs a pass of np momentum slots in the
+// wave-local form (one-wave workgroups)
+inline bool grid_wave_pass(int unit, int np) { return unit == 3 && np <= GRID_MOM_PER_TILE / 2; }
 void launch_meson_grid(const MesonGridArgs &a, int storage, int unit, int tile_sites, size_t blocks,
                        size_t tab_entries, hipStream_t stream);
 void launch_meson_grid_finish(const MesonGridFinish &f, int nblocks, hipStream_t stream);

@@ -230,15 +230,20 @@ typedef struct {
 /* Work decomposition; a zero field takes the library default. */
 typedef struct {
   int sites_per_item; /* sites per work item (rounded up to the tile) */
-  int tile_sites;     /* sites staged per step: 4 or 8 */
-  int blocks_per_cu;  /* persistent workgroups per compute unit: 1..8 */
+  int tile_sites;     /* sites staged per step: 4 or 8; 0: 8 for plain FP32
+                         with vector-ALU traces, else 4 */
+  int blocks_per_cu;  /* persistent workgroups per compute unit: 1..8; 0:
+                         3 four-wave workgroups for matrix-core traces, 2 for
+                         vector-ALU traces, or for a wave-local pass 4
+                         one-wave workgroups (8 for FP32 site-scaled) */
   int trace_unit;     /* per-site traces on 1: the vector ALUs, 2: the matrix
                          cores (FP64 and FP32 site-scaled storage only),
                          3: the matrix cores with one wave per 4-site tile
                          holding all of K (same storage; passes of at most 4
                          momentum slots, other passes run as 2; tile_sites
                          ignored, blocks_per_cu counts one-wave workgroups);
-                         0: the library's choice */
+                         0: the library's choice (2 for FP64, 1 for
+                         FP32, 3 for FP32 site-scaled) */
 } contract_meson_grid_tuning;
 
 size_t contract_meson_grid_workspace_bytes(const contract_meson_request *reqs, int nreq, int nmom,
