@@ -745,6 +745,12 @@ def summarise(requests, wall, pool, batch_sizes, launches) -> Dict[str, object]:
         "meson_batch_mean": float(np.mean(batch_sizes["meson"])) if batch_sizes["meson"] else 0.0,
         "baryon_batch_mean": float(np.mean(batch_sizes["baryon"])) if batch_sizes["baryon"] else 0.0,
         "launches": dict(launches),
+        # the five slowest requests: kind, lattice, latency, queueing before
+        # admission (ms) and arrival (s into the window), to tell a backlog
+        # from one slow launch
+        "slowest": [[r.kind, "x".join(str(d) for d in r.dims), round(1e3 * r.latency, 2),
+                     None if r.admitted is None else round(1e3 * (r.admitted - r.arrival), 2), round(r.arrival, 4)]
+                    for r in sorted((r for r in requests if r.latency is not None), key=lambda r: -r.latency)[:5]],
     }
 
 

@@ -224,8 +224,9 @@ def test_summary_records_realised_arrival_rate():
     from contract_amd import serving
 
     class R:
-        def __init__(self, t):
-            self.arrival, self.done, self.latency, self.kind = t, t + 0.01, 0.01, "meson"
+        def __init__(self, t, lat=0.01):
+            self.arrival, self.done, self.latency, self.kind = t, t + lat, lat, "meson"
+            self.dims, self.admitted = (4, 4, 4, 8), t + 0.002
 
     class P:
         cap = peak = hits = misses = evictions = 0
@@ -234,3 +235,8 @@ def test_summary_records_realised_arrival_rate():
     out = serving.summarise(reqs, 1.1, P(), {"meson": [1], "baryon": []}, {})
     assert abs(out["arrival_rps"] - 10.0) < 1e-9
     assert serving.summarise([R(0.0), R(0.0)], 0.1, P(), {"meson": [], "baryon": []}, {})["arrival_rps"] is None
+    # the slowest requests, slowest first, with their queueing time
+    reqs[3] = R(0.3, lat=0.5)
+    slow = serving.summarise(reqs, 1.1, P(), {"meson": [1], "baryon": []}, {})["slowest"]
+    assert len(slow) == 5 and slow[0] == ["meson", "4x4x4x8", 500.0, 2.0, 0.3]
+    assert all(a[2] >= b[2] for a, b in zip(slow, slow[1:]))
