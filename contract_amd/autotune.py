@@ -201,9 +201,10 @@ def candidates(sites: Sequence[int] = DEFAULT_SITES, tiles: Sequence[int] = DEFA
 
 
 def _units_allowed(requests) -> Sequence[int]:
-    """Plain complex64 storage has no matrix-core trace form."""
+    """Plain complex64 storage has no shared-tile matrix-core form (its
+    wave-local form scales each site as the tile lands)."""
     import torch
-    return (1,) if getattr(requests[0][0], "dtype", None) == torch.complex64 else DEFAULT_UNITS
+    return (1, 3) if getattr(requests[0][0], "dtype", None) == torch.complex64 else DEFAULT_UNITS
 
 
 def _time(fn, stream, reps: int, warmup: int) -> float:

@@ -26,9 +26,10 @@ struct GridTuning {
 
 // Library defaults, each the fastest plan for its storage on a batch of 24
 // lattices from 16^3x32 to 32^3x64 at 7 momenta: 1024 sites per item; FP64
-// the shared-tile matrix-core form, tile 4; plain FP32 the vector-ALU form,
-// tile 8; site-scaled FP32 the wave-local form.  A zero blocks_per_cu takes
-// the default of the form each pass runs (pass_blocks_per_cu).
+// the shared-tile matrix-core form, tile 4; both FP32 forms the wave-local
+// form (plain FP32 scaled per site in the kernel).  An explicit vector-ALU
+// plan for plain FP32 defaults to tile 8.  A zero blocks_per_cu takes the
+// default of the form each pass runs (pass_blocks_per_cu).
 GridTuning resolve_tuning(const contract_meson_grid_tuning *t, int precision)
 {
   GridTuning g{1024, 0, 0, 0};
@@ -38,19 +39,19 @@ GridTuning resolve_tuning(const contract_meson_grid_tuning *t, int precision)
     g.bpc = t->blocks_per_cu;
     g.unit = t->trace_unit;
   }
-  if (g.unit == 0) g.unit = precision == CONTRACT_FP32 ? 1 : (precision == CONTRACT_FP32_SCALED ? 3 : 2);
+  if (g.unit == 0) g.unit = precision == CONTRACT_FP64 ? 2 : 3;
   if (g.tile == 0) g.tile = precision == CONTRACT_FP32 && g.unit == 1 ? 8 : 4;
   return g;
 }
 
 // Persistent workgroups per CU of a pass: the one-wave workgroups of the
-// wave-local form fill one SIMD each (two for site-scaled storage, whose
-// form fits two waves per SIMD); three four-wave workgroups for the
+// wave-local form fill one SIMD each (two for the FP32 forms, which fit two
+// waves per SIMD); three four-wave workgroups for the
 // shared-tile matrix-core form, two for the vector-ALU form.
 int pass_blocks_per_cu(const GridTuning &g, bool wave_pass, int precision)
 {
   if (g.bpc) return g.bpc;
-  if (wave_pass) return precision == CONTRACT_FP32_SCALED ? 8 : 4;
+  if (wave_pass) return precision == CONTRACT_FP64 ? 4 : 8;
   return g.unit >= 2 ? 3 : 2;
 }
 
@@ -60,8 +61,9 @@ int check_tuning(const GridTuning &g, int precision)
     set_last_error("trace_unit = %d is not 0, 1, 2 or 3", g.unit);
     return CONTRACT_ERR_ARG;
   }
-  if (g.unit >= 2 && precision == CONTRACT_FP32) {
-    set_last_error("trace_unit = %d (matrix cores) needs CONTRACT_FP64 or CONTRACT_FP32_SCALED storage", g.unit);
+  if (g.unit == 2 && precision == CONTRACT_FP32) {
+    set_last_error("trace_unit = 2 (shared-tile matrix-core traces) needs CONTRACT_FP64 or CONTRACT_FP32_SCALED "
+                   "storage; plain CONTRACT_FP32 takes 1 or 3");
     return CONTRACT_ERR_ARG;
   }
   if (g.tile != 4 && g.tile != 8) {

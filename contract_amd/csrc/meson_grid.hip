@@ -509,10 +509,62 @@ template <typename R> __device__ __forceinline__ void transpose_rows4(R &r0, R &
     }                                                                                                       \
   } while (0)
 
-template <typename Real, bool SCALED, bool DMA>
+template <int CTRL> __device__ __forceinline__ float dpp_move(float x)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
+}
+
+// Maximum over each 16-lane row, returned to every lane of the row: DPP
+// quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, row_mirror.
+__device__ __forceinline__ float row_max16(float v)
+{
+  v = fmaxf(v, dpp_move<0xb1>(v));
+  v = fmaxf(v, dpp_move<0x4e>(v));
+  v = fmaxf(v, dpp_move<0x141>(v));
+  return fmaxf(v, dpp_move<0x140>(v));
+}
+
+// Plain FP32 storage in the wave-local form: for lane row g, e = ilogb of
+// the largest entry of site g of a staged 4-site tile (the per-site exponent
+// that FP32 site-scaled storage holds: 0 for an all-zero site, within FP32's
+// normal range).
+__device__ __forceinline__ int site_exponent(const float2 *tile, int lane)
+{
+  constexpr int SLOTS = SITE_ELEMS / 2; // 16-byte slots per site
+  constexpr int M = (SLOTS + 15) / 16;
+  const int g = lane >> 4, q = lane & 15;
+  const float4 *site = reinterpret_cast<const float4 *>(tile + g * SITE_ELEMS);
+  float mx = 0.0f;
+#pragma unroll
+  for (int m = 0; m < M; ++m)
+    if (m + 1 < M || q + 16 * m < SLOTS) {
+      const float4 v = site[q + 16 * m];
+      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+    }
+  mx = row_max16(mx);
+  const int e = mx > 0.0f ? ilogbf(mx) : 0;
+  return e < -126 ? -126 : (e > 126 ? 126 : e);
+}
+
+// 2^-e of site sk (held by lane row sk) for the whole wave
+__device__ __forceinline__ float site_scale(float sc, int sk)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sc), 16 * sk));
+}
+
+// AUTO: plain FP32 storage, scaled in the kernel.  Once a tile has landed,
+// lane row g finds the exponents of site g (site_exponent); the traces
+// scale every operand of a site by 2^-e of its propagator as they read it,
+// so their FP32 products stay far above the underflow threshold however far
+// the propagators have decayed, and the operands are the values FP32
+// site-scaled storage holds (exact powers of two: the results equal that
+// form's bit for bit).  The results are scaled back in FP64 as for
+// site-scaled storage.
+template <typename Real, bool SCALED, bool DMA, bool AUTO = false>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(DMA ? 2 : 1, DMA ? 2 : 1))) void
 meson_grid_wave_kernel(MesonGridArgs a)
 {
+  static_assert(!AUTO || (DMA && !SCALED && sizeof(Real) == 4), "in-kernel scaling is a form of the FP32 DMA kernel");
   using C = cplx<Real>;
   using T = Tile<C>;
   using KAcc = typename std::conditional<sizeof(Real) == 8, dbl4, flt4>::type;
@@ -603,8 +655,17 @@ meson_grid_wave_kernel(MesonGridArgs a)
     for (size_t st = first; st < last; st += SITES_PER_WAVE) {
       const size_t next = st + SITES_PER_WAVE;
       const int nnext = next < last ? tile_n(next) : 0; // an empty range past the item
+      int ex_auto = 0;               // AUTO: lane row g's site exponents, P_i's plus P_j's
+      float sci = 1.0f, scj = 1.0f; // AUTO: 2^-exponent of lane row g's site
       if constexpr (DMA) {
         dma_wait(); // this tile's loads, the only ones in flight here
+        if constexpr (AUTO) {
+          const int ei = site_exponent(reinterpret_cast<const float2 *>(dma_i), lane);
+          const int ej = site_exponent(reinterpret_cast<const float2 *>(dma_j), lane);
+          ex_auto = ei + ej;
+          sci = ldexpf(1.0f, -ei);
+          scj = ldexpf(1.0f, -ej);
+        }
       } else {
         wave_sync(); // the previous step's readers of the tile are done
         bi.store(lds[0], lane);
@@ -620,10 +681,16 @@ meson_grid_wave_kernel(MesonGridArgs a)
       for (int sk = 0; sk < SITES_PER_WAVE; ++sk) {
         const C *xs = tile_i + sk * SL, *ys = tile_j + sk * SL;
         KAcc kr = KAcc{0, 0, 0, 0}, ki = KAcc{0, 0, 0, 0}, t3 = KAcc{0, 0, 0, 0};
+        Real si = 1, sj = 1; // AUTO: this site's scales
+        if constexpr (AUTO) {
+          si = site_scale(sci, sk);
+          sj = site_scale(scj, sk);
+        }
 #pragma unroll
         for (int ch = 0; ch < 2; ++ch) {
           const int cc = 4 * ch + g;
-          const C xv = xs[q * 9 + cc], yv = ys[q * 9 + cc];
+          const C xv = AUTO ? cscale(xs[q * 9 + cc], si) : xs[q * 9 + cc];
+          const C yv = AUTO ? cscale(ys[q * 9 + cc], sj) : ys[q * 9 + cc];
           if constexpr (sizeof(Real) == 8) {
             kr = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.x, yv.x, kr, 0, 0, 0);
             ki = __builtin_amdgcn_mfma_f64_16x16x4f64(xv.y, yv.y, ki, 0, 0, 0);
@@ -640,11 +707,11 @@ meson_grid_wave_kernel(MesonGridArgs a)
           kr = t1 + t2;
           ki = t3 - t1 + t2;
         }
-        const C yv = ys[q * 9 + 8]; // colour pair 8 on the vector ALUs
+        const C yv = AUTO ? cscale(ys[q * 9 + 8], sj) : ys[q * 9 + 8]; // colour pair 8 on the vector ALUs
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = sizeof(Real) == 8 ? g + 4 * v : 4 * g + v;
-          const C t = cfma_conj(cmake<C>(kr[v], ki[v]), xs[row * 9 + 8], yv);
+          const C t = cfma_conj(cmake<C>(kr[v], ki[v]), AUTO ? cscale(xs[row * 9 + 8], si) : xs[row * 9 + 8], yv);
           kr[v] = t.x;
           ki[v] = t.y;
         }
@@ -663,6 +730,7 @@ meson_grid_wave_kernel(MesonGridArgs a)
       if constexpr (DMA) {
         GRID_SITE_TERMS(ex, av4);
       }
+      if constexpr (AUTO) ex = ex_auto; // lane row g's site
       if constexpr (DMA) { // the tile has been read: the next one into its place
         wave_sync();
         // past the item an empty range (its zeros land before the next
@@ -690,7 +758,7 @@ meson_grid_wave_kernel(MesonGridArgs a)
         for (int v = 0; v < 4; ++v) {
           const int ik = sizeof(Real) == 8 ? gg + 4 * v : 4 * gg + v;
           double re = (double)Kr[gg][v], im = (double)Ki[gg][v];
-          if constexpr (SCALED) {
+          if constexpr (SCALED || AUTO) {
             re = ldexp(re, ex);
             im = ldexp(im, ex);
           }
@@ -758,6 +826,9 @@ void launch_meson_grid(const MesonGridArgs &a, int storage, int unit, int tile_s
     if (storage == GRID_FP32_SCALED)
       hipLaunchKernelGGL((meson_grid_wave_kernel<float, true, true>), dim3((unsigned)blocks), dim3(WAVE), shmem,
                          stream, a);
+    else if (storage == GRID_FP32)
+      hipLaunchKernelGGL((meson_grid_wave_kernel<float, false, true, true>), dim3((unsigned)blocks), dim3(WAVE),
+                         shmem, stream, a);
     else
       hipLaunchKernelGGL((meson_grid_wave_kernel<double, false, false>), dim3((unsigned)blocks), dim3(WAVE), shmem,
                          stream, a);

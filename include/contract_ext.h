@@ -235,15 +235,17 @@ typedef struct {
   int blocks_per_cu;  /* persistent workgroups per compute unit: 1..8; 0:
                          3 four-wave workgroups for matrix-core traces, 2 for
                          vector-ALU traces, or for a wave-local pass 4
-                         one-wave workgroups (8 for FP32 site-scaled) */
+                         one-wave workgroups (8 for the FP32 forms) */
   int trace_unit;     /* per-site traces on 1: the vector ALUs, 2: the matrix
                          cores (FP64 and FP32 site-scaled storage only),
                          3: the matrix cores with one wave per 4-site tile
-                         holding all of K (same storage; passes of at most 4
-                         momentum slots, other passes run as 2; tile_sites
-                         ignored, blocks_per_cu counts one-wave workgroups);
-                         0: the library's choice (2 for FP64, 1 for
-                         FP32, 3 for FP32 site-scaled) */
+                         holding all of K (any storage; plain FP32 tiles are
+                         scaled per site as they land; passes of at most 4
+                         momentum slots, other passes run as 2, or as 1 for
+                         plain FP32; tile_sites ignored, blocks_per_cu
+                         counts one-wave workgroups);
+                         0: the library's choice (2 for FP64, 3 for both
+                         FP32 forms) */
 } contract_meson_grid_tuning;
 
 size_t contract_meson_grid_workspace_bytes(const contract_meson_request *reqs, int nreq, int nmom,

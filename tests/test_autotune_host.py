@@ -68,6 +68,9 @@ def test_candidates_hold_the_defaults():
     assert (1024, 4, 2, 2) in c and (1024, 4, 4, 3) in c
     assert len(c) == len(set(c)) == 5 * 2 * 4 * 2 + 5 * 2
     assert all(u != 3 for *_, u in autotune.candidates(units=(1, 2)))
+    # plain FP32: vector-ALU plans and the wave-local form, no shared-tile MFMA form
+    c13 = autotune.candidates(units=(1, 3))
+    assert {u for *_, u in c13} == {1, 3} and (1024, 4, 8, 3) in c13 and (1024, 8, 2, 1) in c13
 
 
 def test_cache_file_round_trip(tmp_path, monkeypatch):

@@ -54,6 +54,26 @@ def test_meson_grid_tuning(tuner):
     assert again == best and list(t3) == [best]
 
 
+def test_meson_grid_tuning_plain_fp32(tuner):
+    """Plain complex64 storage: the tuner times the vector-ALU and the
+    wave-local forms (no shared-tile matrix-core form) and the winner's
+    results are within 1e-6 of the oracle on the same complex64 values."""
+    torch, pc, autotune, cuda = tuner
+    lats = [(8, 8, 8, 8), (4, 4, 4, 16)]
+    host = [(oracle.synthetic_propagator(d, 40 + k, 0.3).astype(np.complex64),
+             oracle.synthetic_propagator(d, 50 + k, 0.3).astype(np.complex64)) for k, d in enumerate(lats)]
+    reqs = [(torch.from_numpy(a).to(cuda), torch.from_numpy(b).to(cuda), d, 0) for (a, b), d in zip(host, lats)]
+    units = autotune._units_allowed(reqs)
+    assert set(units) == {1, 3}
+    cands = autotune.candidates(sites=(256, 1024), tiles=(4, 8), blocks=(2,), units=units)
+    best, table = autotune.tune_meson_grid(reqs, MOM7, cands=cands, reps=2)
+    assert set(table) == set(cands) and {u for *_, u in table} == {1, 3}
+    outs = pc.mesonGridProject(reqs, MOM7, tuning=best)
+    for (a, b), d, o in zip(host, lats, outs):
+        want = oracle.meson_grid_project(a.astype(np.complex128), b.astype(np.complex128), d, MOM7)
+        assert per_timeslice_error(o.cpu().numpy(), want) < 1e-6
+
+
 def test_baryon_plan_tuning(tuner):
     torch, pc, autotune, cuda = tuner
     dims = (8, 8, 8, 16)

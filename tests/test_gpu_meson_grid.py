@@ -336,3 +336,53 @@ def test_wave_local_equals_shared_tile_form_to_rounding(dev):
     for r, m in zip(reqs, three):
         alone = pc.mesonGridProject([r], MOM7, tuning=(16, 4, 4, 3))[0].cpu().numpy()
         assert np.array_equal(alone, m)
+
+
+@pytest.mark.parametrize("waves", [1, 8])
+def test_plain_fp32_wave_local_matches_oracle(dev, waves):
+    """Plain complex64 storage through trace_unit 3 (each site scaled by the
+    power of two of its largest entry as the tile lands, FP32 matrix-core
+    traces, FP64 projection): an uneven batch with ragged items in both site
+    orders and 1-7 momenta within 1e-6 of the FP64 oracle on the same
+    complex64 values, and more than 4 slots (those passes on the vector
+    ALUs) too."""
+    torch, pc, _ = dev
+    reqs, hosts = uneven_batch(dev, np.complex64)
+    for order, dims in ((0, (3, 3, 3, 4)), (1, (6, 3, 3, 4))):
+        r, h = make_request(dev, dims, 78 + order, 1, order, np.complex64)
+        reqs.append(r)
+        hosts.append(h)
+    for chunk in (12, 16):
+        for moms in (MOM7, MOM7[:1], MOM7[:3]):
+            outs = pc.mesonGridProject(reqs, moms, tuning=(chunk, 4, waves, 3))
+            for out, hh in zip(outs, hosts):
+                assert timeslice_error(out.cpu().numpy(), oracle_grid(hh, moms)) < 1e-6, (chunk, moms)
+    outs = pc.mesonGridProject(reqs[:2], MOM27, tuning=(16, 4, waves, 3))
+    for out, hh in zip(outs, hosts[:2]):
+        assert timeslice_error(out.cpu().numpy(), oracle_grid(hh, MOM27)) < 1e-6
+
+
+def test_plain_fp32_wave_local_equals_site_scaled(dev):
+    """In-kernel per-site scaling of complex64 tiles gives the tiles that FP32
+    site-scaled storage of the same values holds (same exponent rule, exact
+    powers of two), so trace_unit 3 returns the same bits for both storage
+    forms."""
+    torch, pc, _ = dev
+    reqs, _ = uneven_batch(dev, np.complex64)
+    plain = [o.cpu().numpy() for o in pc.mesonGridProject(reqs, MOM7, tuning=(16, 4, 8, 3))]
+    scaled = [o.cpu().numpy() for o in pc.mesonGridProject(scaled_batch(dev, reqs), MOM7, tuning=(16, 4, 8, 3))]
+    for a, b in zip(plain, scaled):
+        assert np.array_equal(a, b)
+
+
+def test_plain_fp32_wave_local_steep_decay(dev):
+    """exp(-0.8 |t - t0|) over 64 timeslices in plain complex64: the far
+    correlator (~1e-44) is below FP32's normal range, the FP32 products of
+    unscaled entries would underflow; the per-site scaling keeps it within
+    1e-6 of the oracle."""
+    torch, pc, _ = dev
+    req, host = make_request(dev, (2, 2, 2, 64), 41, 0, dtype=np.complex64, alpha=0.8)
+    got = pc.mesonGridProject([req], MOM7[:1], tuning=(8, 4, 8, 3))[0].cpu().numpy()
+    want = oracle_grid(host, MOM7[:1])
+    assert np.abs(want[:, :, 0, -1]).max() < 1e-40
+    assert timeslice_error(got, want) < 1e-6
