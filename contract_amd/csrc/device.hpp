@@ -273,6 +273,35 @@ __device__ __forceinline__ void dma_tile(C *__restrict__ lds, const C *__restric
                                              BUFFER_AUX_NT); // lane offset in a VGPR, instruction offset in an SGPR
 }
 
+// dma_tile of the same sites of two propagators, its instructions spread
+// over the NW waves of a workgroup (wave w issues instructions w, w + NW, ...
+// of each propagator).  Each wave waits for its own share (dma_wait); a
+// workgroup barrier after that makes the whole tile visible.  Every
+// instruction names its destination array directly, so the compiler can
+// tell reads of another array from these loads.
+template <int NW, int NSITE, typename C>
+__device__ __forceinline__ void dma_tile_pair_split(C *__restrict__ lds_i, C *__restrict__ lds_j,
+                                                    const C *__restrict__ src_i, const C *__restrict__ src_j,
+                                                    size_t first_site, int nsites, int lane, int wave)
+{
+  using T = Tile<C, NSITE>;
+  typedef __attribute__((address_space(3))) void lds_void;
+  const unsigned bytes = (unsigned)(nsites * T::SITE16 * 16);
+  const __amdgpu_buffer_rsrc_t ri = tile_rsrc(src_i + first_site * SITE_ELEMS, bytes);
+  const __amdgpu_buffer_rsrc_t rj = tile_rsrc(src_j + first_site * SITE_ELEMS, bytes);
+  uint4 *li = reinterpret_cast<uint4 *>(lds_i), *lj = reinterpret_cast<uint4 *>(lds_j);
+#pragma unroll
+  for (int m = 0; m < (T::PER_LANE + NW - 1) / NW; ++m) {
+    const int it = wave + m * NW;
+    if (it < T::PER_LANE) {
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(ri, (lds_void *)(li + it * WAVE), 16, lane * 16, it * WAVE * 16, 0,
+                                               BUFFER_AUX_NT);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rj, (lds_void *)(lj + it * WAVE), 16, lane * 16, it * WAVE * 16, 0,
+                                               BUFFER_AUX_NT);
+    }
+  }
+}
+
 // Orders the LDS traffic between the lanes of a one-wave workgroup (a wave's
 // LDS instructions execute in issue order, so finishing the earlier ones is
 // enough).  __syncthreads() is a workgroup release fence, which also waits for

@@ -254,20 +254,26 @@ int contract_meson_grid_project(const contract_meson_request *reqs, int nreq, in
       tab = std::max(tab, q.dims[0] + q.dims[1] + q.dims[2] + 3);
     }
     a.total_items = (unsigned)items;
+    const int storage =
+        precision == CONTRACT_FP32_SCALED ? GRID_FP32_SCALED : (precision == CONTRACT_FP32 ? GRID_FP32 : GRID_FP64);
     for (size_t p0 = 0; p0 < slots.size(); p0 += GRID_MOM_PER_PASS) {
       a.np = (int)std::min(slots.size() - p0, (size_t)GRID_MOM_PER_PASS);
-      const size_t grid =
-          std::min(items, (size_t)device_cu_count() * pass_blocks_per_cu(g, grid_wave_pass(g.unit, a.np), precision));
+      int bpc = pass_blocks_per_cu(g, grid_wave_pass(g.unit, a.np), precision);
+      if (!g.bpc) {
+        // a default plan runs no more persistent workgroups than fit on a
+        // CU at once: the rest would start only as resident ones finish
+        // their whole item lists
+        const int fit = meson_grid_resident_blocks(a, storage, g.unit, g.tile, (size_t)tab * a.np);
+        if (fit > 0 && fit < bpc) bpc = fit;
+      }
+      const size_t grid = std::min(items, (size_t)device_cu_count() * bpc);
       for (int p = 0; p < a.np; ++p) {
         const MomSlot &m = slots[p0 + p];
         for (int mu = 0; mu < 3; ++mu) a.mom[p][mu] = momenta[3 * m.rep + mu];
         f.out_plus[p] = m.plus;
         f.out_minus[p] = m.minus;
       }
-      launch_meson_grid(a,
-                        precision == CONTRACT_FP32_SCALED ? GRID_FP32_SCALED
-                                                          : (precision == CONTRACT_FP32 ? GRID_FP32 : GRID_FP64),
-                        g.unit, g.tile, grid, (size_t)tab * a.np, stream);
+      launch_meson_grid(a, storage, g.unit, g.tile, grid, (size_t)tab * a.np, stream);
       CONTRACT_EXT_HIP(hipGetLastError());
       f.np = a.np;
       launch_meson_grid_finish(f, blocks, stream);

@@ -78,16 +78,28 @@ __device__ __forceinline__ int scale_block(float2 (&b)[9])
 // 16i + 4b + j.  With k = site g and 4b + j = column q, B is the 16x16x4
 // form's operand unchanged; A is the phase of row 4 half + (q & 3) at site
 // g; lane (g, q) ends with rows 4 half + g of column q.
+// FP64 matrix-core traces with two row tiles (passes of 9-16 momentum
+// slots): the tile moves HBM -> LDS directly (dma_tile_pair_split, unpadded
+// sites), issued once every wave has traced the current one, which drops the
+// registers of the prefetch and fits two waves per SIMD.
+template <typename Real, bool KMFMA, int RT> constexpr bool grid_dma()
+{
+  return KMFMA && RT == 2 && sizeof(Real) == 8;
+}
+
 template <typename Real, int NS, bool SCALED = false, bool KMFMA = false, int RT = 1, bool Q4 = false>
 __global__ __launch_bounds__(GRID_THREADS)
-__attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !KMFMA)) ? 1 : 2))) void meson_grid_kernel(
-    MesonGridArgs a)
+__attribute__((amdgpu_waves_per_eu(grid_dma<Real, KMFMA, RT>() ? 2
+                                   : ((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !KMFMA)) ? 1 : 2)))) void
+meson_grid_kernel(MesonGridArgs a)
 {
   static_assert(NS % SITES_PER_WAVE == 0, "tile is a whole number of MFMA site steps");
   static_assert(!Q4 || (KMFMA && RT == 1), "the quarter projection is a form of the one-tile matrix-core kernel");
   using C = cplx<Real>;
   using T = Tile<C>;
-  constexpr int SL = T::LDS_ELEMS;               // padded site stride (complex entries)
+  constexpr bool DMA = grid_dma<Real, KMFMA, RT>();
+  static_assert(!DMA || Tile<C, NS>::DMA_ELEMS == NS * SITE_ELEMS, "a DMA tile is whole instructions");
+  constexpr int SL = DMA ? SITE_ELEMS : T::LDS_ELEMS; // site stride (complex entries): padded unless DMA
   constexpr int CH = NS * T::SITE16;             // 16-byte chunks per propagator tile
   constexpr int PER_PROP = (CH + GRID_THREADS - 1) / GRID_THREADS; // chunks per thread and propagator
   constexpr int TILE16 = NS * T::LDS16;          // one propagator tile in LDS, 16-byte units
@@ -224,12 +236,52 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
           l16[TILE16 + dst_off[it]] = buf[1][it];
         }
     };
-    load(first);
+    auto tile_sites = [&](size_t st) {
+      const size_t left = last - st;
+      return left < (size_t)NS ? (int)left : NS;
+    };
+    if constexpr (DMA)
+      dma_tile_pair_split<GRID_THREADS / WAVE, NS>(lds[0], lds[1], P0, P1, first, tile_sites(first), lane, w);
+    else
+      load(first);
+    // phases of this lane's site in four-site group h of the step (sites past
+    // the item's end carry K = 0; their coordinates only need to stay inside
+    // the tables)
+    auto site_phases = [&](int h, double (&av)[RT], double (&av4)[2]) {
+      const int ls = h * SITES_PER_WAVE + g;
+      int xr = x0 + ls, y = y0, z = z0;
+      while (xr >= Lr) {
+        xr -= Lr;
+        if (++y == Ly) {
+          y = 0;
+          ++z;
+        }
+      }
+      z = z < Lz ? z : 0;
+      const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
+      auto phase = [&](int pr) {
+        const double2 ph =
+          cmul(cmul(tab[pr * (Lx + 1) + x], tab[offy + pr * (Ly + 1) + y]), tab[offz + pr * (Lz + 1) + z]);
+        return (q & 1) ? ph.y : ph.x;
+      };
+      if constexpr (Q4) {
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) av4[hf] = live4[hf] ? phase(prow4[hf]) : 0.0;
+      } else {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) av[rt] = live[rt] ? phase(prow[rt]) : 0.0;
+      }
+    };
     for (size_t st = first; st < last; st += NS) {
-      __syncthreads(); // the readers of the previous tile are done
-      store();
-      if (st + NS < last) load(st + NS);
-      __syncthreads();
+      if constexpr (DMA) {
+        dma_wait();      // this wave's share of the tile
+        __syncthreads(); // every share has landed; the readers of the previous K are done
+      } else {
+        __syncthreads(); // the readers of the previous tile are done
+        store();
+        if (st + NS < last) load(st + NS);
+        __syncthreads();
+      }
       if constexpr (KMFMA) {
         // traces of this wave's sites of the tile, into kbuf
 #pragma unroll
@@ -297,33 +349,28 @@ __attribute__((amdgpu_waves_per_eu((RT > 1 || (sizeof(Real) == 8 && NS == 8 && !
         }
         __syncthreads();
       }
+      // DMA: the phases of every site group (table reads before the next
+      // tile's loads are issued: a dynamic-LDS read after them would wait for
+      // them), then the next tile into the traced one's place
+      double avd[DMA ? NS / SITES_PER_WAVE : 1][RT];
+      if constexpr (DMA) {
+#pragma unroll
+        for (int h = 0; h < NS / SITES_PER_WAVE; ++h) {
+          double unused4[2];
+          site_phases(h, avd[h], unused4);
+        }
+        if (st + NS < last)
+          dma_tile_pair_split<GRID_THREADS / WAVE, NS>(lds[0], lds[1], P0, P1, st + NS, tile_sites(st + NS), lane, w);
+      }
 #pragma unroll
       for (int h = 0; h < NS / SITES_PER_WAVE; ++h) {
         const int ls = h * SITES_PER_WAVE + g; // this lane's site in the tile
-        // coordinates of site st + ls (sites past the item's end carry K = 0;
-        // their coordinates only need to stay inside the tables)
-        int xr = x0 + ls, y = y0, z = z0;
-        while (xr >= Lr) {
-          xr -= Lr;
-          if (++y == Ly) {
-            y = 0;
-            ++z;
-          }
-        }
-        z = z < Lz ? z : 0;
-        const int x = R.order ? 2 * xr + ((pbase + y + z) & 1) : xr;
-        auto phase = [&](int pr) {
-          const double2 ph =
-            cmul(cmul(tab[pr * (Lx + 1) + x], tab[offy + pr * (Ly + 1) + y]), tab[offz + pr * (Lz + 1) + z]);
-          return (q & 1) ? ph.y : ph.x;
-        };
         double av[RT], av4[2];
-        if constexpr (Q4) {
+        if constexpr (DMA) {
 #pragma unroll
-          for (int hf = 0; hf < 2; ++hf) av4[hf] = live4[hf] ? phase(prow4[hf]) : 0.0;
+          for (int rt = 0; rt < RT; ++rt) av[rt] = avd[h][rt];
         } else {
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt) av[rt] = live[rt] ? phase(prow[rt]) : 0.0;
+          site_phases(h, av, av4);
         }
         if constexpr (KMFMA) {
           // rows ik = 4 i + k of this wave's k = w; B[site g][jl = q]
@@ -795,58 +842,72 @@ meson_grid_wave_kernel(MesonGridArgs a)
 
 #undef GRID_SITE_TERMS
 
-template <typename Real, bool SCALED, bool KMFMA, int RT, bool Q4 = false>
-static void launch_grid_rt(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
+// The kernel a pass runs, handed to f(kernel, threads per workgroup): the
+// launch and the occupancy query below select it the same way.
+template <typename Real, bool SCALED, bool KMFMA, int RT, bool Q4 = false, typename F>
+static void with_grid_rt(int tile_sites, F &&f)
 {
   if (tile_sites == 8)
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 8, SCALED, KMFMA, RT, Q4>), dim3((unsigned)blocks),
-                       dim3(GRID_THREADS), shmem, stream, a);
+    f(meson_grid_kernel<Real, 8, SCALED, KMFMA, RT, Q4>, GRID_THREADS);
   else
-    hipLaunchKernelGGL((meson_grid_kernel<Real, 4, SCALED, KMFMA, RT, Q4>), dim3((unsigned)blocks),
-                       dim3(GRID_THREADS), shmem, stream, a);
+    f(meson_grid_kernel<Real, 4, SCALED, KMFMA, RT, Q4>, GRID_THREADS);
 }
 
-template <typename Real, bool SCALED, bool KMFMA>
-static void launch_grid_t(const MesonGridArgs &a, int tile_sites, size_t blocks, size_t shmem, hipStream_t stream)
+template <typename Real, bool SCALED, bool KMFMA, typename F>
+static void with_grid_t(const MesonGridArgs &a, int tile_sites, F &&f)
 {
   if (a.np > GRID_MOM_PER_TILE)
-    launch_grid_rt<Real, SCALED, KMFMA, 2>(a, tile_sites, blocks, shmem, stream);
+    with_grid_rt<Real, SCALED, KMFMA, 2>(tile_sites, f);
   else if (KMFMA && a.np <= GRID_MOM_PER_TILE / 2)
-    launch_grid_rt<Real, SCALED, KMFMA, 1, KMFMA>(a, tile_sites, blocks, shmem, stream);
+    with_grid_rt<Real, SCALED, KMFMA, 1, KMFMA>(tile_sites, f);
   else
-    launch_grid_rt<Real, SCALED, KMFMA, 1>(a, tile_sites, blocks, shmem, stream);
+    with_grid_rt<Real, SCALED, KMFMA, 1>(tile_sites, f);
+}
+
+template <typename F> static void with_grid_kernel(const MesonGridArgs &a, int storage, int unit, int tile_sites, F &&f)
+{
+  const bool mfma_traces = unit >= 2;
+  if (grid_wave_pass(unit, a.np)) { // wave-local K; other passes take the shared-tile form
+    if (storage == GRID_FP32_SCALED)
+      f(meson_grid_wave_kernel<float, true, true>, WAVE);
+    else if (storage == GRID_FP32)
+      f(meson_grid_wave_kernel<float, false, true, true>, WAVE);
+    else
+      f(meson_grid_wave_kernel<double, false, false>, WAVE);
+    return;
+  }
+  if (storage == GRID_FP32_SCALED) {
+    if (mfma_traces)
+      with_grid_t<float, true, true>(a, tile_sites, f);
+    else
+      with_grid_t<float, true, false>(a, tile_sites, f);
+  } else if (storage == GRID_FP32) {
+    with_grid_t<float, false, false>(a, tile_sites, f);
+  } else {
+    if (mfma_traces)
+      with_grid_t<double, false, true>(a, tile_sites, f);
+    else
+      with_grid_t<double, false, false>(a, tile_sites, f);
+  }
 }
 
 void launch_meson_grid(const MesonGridArgs &a, int storage, int unit, int tile_sites, size_t blocks,
                        size_t tab_entries, hipStream_t stream)
 {
   const size_t shmem = tab_entries * sizeof(double2);
-  const bool mfma_traces = unit >= 2;
-  if (grid_wave_pass(unit, a.np)) { // wave-local K; other passes take the shared-tile form
-    if (storage == GRID_FP32_SCALED)
-      hipLaunchKernelGGL((meson_grid_wave_kernel<float, true, true>), dim3((unsigned)blocks), dim3(WAVE), shmem,
-                         stream, a);
-    else if (storage == GRID_FP32)
-      hipLaunchKernelGGL((meson_grid_wave_kernel<float, false, true, true>), dim3((unsigned)blocks), dim3(WAVE),
-                         shmem, stream, a);
-    else
-      hipLaunchKernelGGL((meson_grid_wave_kernel<double, false, false>), dim3((unsigned)blocks), dim3(WAVE), shmem,
-                         stream, a);
-    return;
-  }
-  if (storage == GRID_FP32_SCALED) {
-    if (mfma_traces)
-      launch_grid_t<float, true, true>(a, tile_sites, blocks, shmem, stream);
-    else
-      launch_grid_t<float, true, false>(a, tile_sites, blocks, shmem, stream);
-  } else if (storage == GRID_FP32) {
-    launch_grid_t<float, false, false>(a, tile_sites, blocks, shmem, stream);
-  } else {
-    if (mfma_traces)
-      launch_grid_t<double, false, true>(a, tile_sites, blocks, shmem, stream);
-    else
-      launch_grid_t<double, false, false>(a, tile_sites, blocks, shmem, stream);
-  }
+  with_grid_kernel(a, storage, unit, tile_sites, [&](auto kernel, int threads) {
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(threads), shmem, stream, a);
+  });
+}
+
+int meson_grid_resident_blocks(const MesonGridArgs &a, int storage, int unit, int tile_sites, size_t tab_entries)
+{
+  const size_t shmem = tab_entries * sizeof(double2);
+  int n = 0;
+  with_grid_kernel(a, storage, unit, tile_sites, [&](auto kernel, int threads) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, shmem) != hipSuccess) n = 0;
+  });
+  return n;
 }
 
 __device__ __forceinline__ void nsum(double &s, double &c, double v)

@@ -73,6 +73,10 @@ enum GridStorage { GRID_FP64 = 0, GRID_FP32 = 1, GRID_FP32_SCALED = 2 };
 inline bool grid_wave_pass(int unit, int np) { return unit == 3 && np <= GRID_MOM_PER_TILE / 2; }
 void launch_meson_grid(const MesonGridArgs &a, int storage, int unit, int tile_sites, size_t blocks,
                        size_t tab_entries, hipStream_t stream);
+// Workgroups of the kernel launch_meson_grid would run for this pass that fit
+// on one CU at once (registers, LDS with the phase tables), 0 if the runtime
+// cannot tell.
+int meson_grid_resident_blocks(const MesonGridArgs &a, int storage, int unit, int tile_sites, size_t tab_entries);
 void launch_meson_grid_finish(const MesonGridFinish &f, int nblocks, hipStream_t stream);
 
 } // namespace contract_amd
