@@ -575,7 +575,8 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle, fi
     7 momenta (origin and the six unit vectors), all in one persistent launch.
     The autotuner picks the work decomposition per precision; the tuned batch
     is then timed eagerly and as a captured HIP graph, in FP64 and in the
-    FP32-contraction / FP64-accumulation form."""
+    FP32-contraction / FP64-accumulation form.  The FP64 batch is also timed
+    at the 27 momenta of {-1,0,1}^3 under the library's default plan."""
     from contract_amd import autotune
     rng = np.random.default_rng(2024)
     reqs = []
@@ -633,6 +634,18 @@ def meson_grid_batch(torch, pc, dev, steps, warmup, hip, timer, check_oracle, fi
         }
         if name == "fp64":
             res[name]["traffic"], res[name]["traffic_source"] = measured_traffic("meson_grid_kernel<double,4,q4>")
+            # the same batch at the 27 momenta of {-1,0,1}^3 (14 +-k slots: one
+            # pass with two projection row tiles), library default plan
+            ws27 = torch.empty(pc.meson_grid_workspace_bytes(batch, 27), dtype=torch.uint8, device=dev)
+            out27 = [torch.empty((16, 16, 27, d[3]), dtype=torch.complex128, device=dev) for _, _, d, _ in batch]
+
+            def step27():
+                pc.mesonGridProject(batch, MOM27, outs=out27, workspace=ws27)
+
+            wall27, kern27 = time_steps(torch, step27, max(steps // 2, 5), warmup, hip, timer)
+            res["fp64_mom27"] = {"momenta": 27, "tuning": "library default", "ms_per_batch": wall27,
+                                 "kernel_ms": kern27, "hbm_frac": bps * sites / (kern27 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            del ws27, out27
         outs[name] = [o.cpu().numpy() for o in out]
         if name == "fp64" and check_oracle:
             import oracle
@@ -1062,6 +1075,8 @@ def north_star(kern64, achieved, kern32, kern32s, err32, err32s, err32c, err32sc
     if grid and "fp64" in grid:
         ns["meson_grid"] = {"fp64_hbm_frac": _r(grid["fp64"]["hbm_frac"]),
                             "fp64_kernel_ms": _r(grid["fp64"]["kernel_ms"])}
+        if "fp64_mom27" in grid:
+            ns["meson_grid"]["fp64_mom27_kernel_ms"] = _r(grid["fp64_mom27"]["kernel_ms"])
         for k in ("fp32_scaled", "fp32"):
             if k in grid:
                 ns["meson_grid"][k + "_speedup"] = _r(grid[k].get("speedup_vs_fp64"))
