@@ -230,55 +230,72 @@ __device__ __forceinline__ C baryon_tail_rt(int type, const C (&D)[9], const C *
 // baryon_two_point passes W = G(gamma_mn), baryon_general_two_point passes
 // W = project_mn (include/baryon_general.cuh:62-67).
 // ---------------------------------------------------------------------------
-// Staged like the fused nucleon kernel: two propagators in LDS at a time
-// (18.5 KB per wave instead of 27.8 KB), so eight waves fit per CU, two per
-// SIMD.  P_j is prefetched a tile ahead; P_i and P_n are loaded at the top of
-// the tile, P_n waits in registers while the diquark is formed and then takes
-// P_i's consumed area; the other wave on the SIMD covers the loads.
+// Staged like the fused nucleon kernel, with its load pipeline: two
+// propagators in LDS at a time (18.5 KB per wave instead of 27.8 KB), so
+// eight waves fit per CU, two per SIMD, and every next-tile load in flight
+// across at least half a step.  Each wave walks one contiguous run of tiles.
+// P_j is register-prefetched a tile ahead and staged at the top of the step;
+// P_i moves HBM -> LDS directly (dma_tile) into an unpadded area of its own,
+// issued once the diquark has consumed the current tile; P_n is stored into
+// P_j's consumed area after the diquark and its next tile is loaded into the
+// freed registers at once.  Steps order LDS with wave_sync: a
+// __syncthreads() would also wait for P_i's load.  Past the run the loads
+// take an empty range (no memory is touched; the registers and P_i's area
+// receive zeros), so no load sits on one side of a branch.
 template <int T, int GIJ>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void baryon_site_kernel(
   BaryonSiteArgs a)
 {
   using C = double2;
   constexpr int SL = site_lds<C>();
-  __shared__ C lds[2][SITES_PER_WAVE * SL];
+  __shared__ C lds[SITES_PER_WAVE * SL];  // P_j, then P_n (register-staged, padded sites)
+  __shared__ C lds_i[Tile<C>::DMA_ELEMS]; // P_i (dma_tile's unpadded layout)
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
   const C w = cmake<C>(a.w_re[q], a.w_im[q]);
   const KlLane<C> kl(a.gkl, a.gklT, q);
   const C *Pi = (const C *)a.propag_i, *Pj = (const C *)a.propag_j, *Pn = (const C *)a.propag_n;
   const size_t ntiles = (a.volume + SITES_PER_WAVE - 1) / SITES_PER_WAVE;
+  // this wave's run of tiles [t0, t1): the launch has at most ntiles waves,
+  // so every run holds at least one tile
+  const size_t t0 = ntiles * blockIdx.x / gridDim.x, t1 = ntiles * (blockIdx.x + 1) / gridDim.x;
+  if (t0 >= t1) return;
   auto nsites_of = [&](size_t t) {
     const size_t left = a.volume - t * SITES_PER_WAVE;
     return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
   };
-  C *si = lds[0] + g * SL, *sj = lds[1] + g * SL;
-  Tile<C> bi, bj, bn;
-  size_t tile = blockIdx.x;
-  if (tile < ntiles) bj.load(Pj, tile * SITES_PER_WAVE, nsites_of(tile), lane);
-  for (; tile < ntiles; tile += gridDim.x) {
-    const size_t first = tile * SITES_PER_WAVE;
+  const C *si = lds_i + g * SITE_ELEMS;
+  C *sj = lds + g * SL;
+  Tile<C> bj, bn;
+  bj.load(Pj, t0 * SITES_PER_WAVE, nsites_of(t0), lane);
+  dma_tile(lds_i, Pi, t0 * SITES_PER_WAVE, nsites_of(t0), lane);
+  bn.load(Pn, t0 * SITES_PER_WAVE, nsites_of(t0), lane);
+  for (size_t tile = t0; tile < t1; ++tile) {
+    const size_t first = tile * SITES_PER_WAVE, next = first + SITES_PER_WAVE;
     const int nsites = nsites_of(tile);
-    bi.load(Pi, first, nsites, lane);
-    bn.load(Pn, first, nsites, lane);
-    bi.store(lds[0], lane);
-    bj.store(lds[1], lane);
-    __syncthreads();
-    const size_t next = tile + gridDim.x;
-    if (next < ntiles) bj.load(Pj, next * SITES_PER_WAVE, nsites_of(next), lane);
+    const int nnext = tile + 1 < t1 ? nsites_of(tile + 1) : 0; // an empty range past the run
+    bj.store(lds, lane);
+    wave_sync();
+    bj.load(Pj, next, nnext, lane);
+    // vmcnt(9): everything issued before P_j's loads just issued, P_i's tile
+    // included, has completed
+    dma_wait_keep<Tile<C>::PER_LANE>();
     C D[9];
     if constexpr (GIJ >= 0)
       diquark_tensor_static<GIJ>(D, si, sj, q);
     else
       diquark_tensor(D, si, sj, q, a.gij);
-    __syncthreads();
-    bn.store(lds[0], lane);
-    __syncthreads();
-    const C R = baryon_tail<T, C, false>(D, si, sj, lane, kl);
+    wave_sync(); // every lane's P_i and P_j reads are done
+    dma_tile(lds_i, Pi, next, nnext, lane);
+    bn.store(lds, lane);
+    bn.load(Pn, next, nnext, lane);
+    wave_sync();
+    const C R = baryon_tail<T, C, false>(D, sj, sj, lane, kl); // no LDS scratch with the all-reduce sum
     const C c = group16_allreduce(cmul(w, R));
     if (q == 0 && g < nsites) static_cast<C *>(a.correl)[first + g] = c;
-    __syncthreads();
+    wave_sync(); // the tail's P_n reads are done before the next P_j store
   }
+  dma_wait(); // the last (empty) P_i load has landed before the wave's LDS is released
 }
 
 template <int GIJ>
@@ -467,6 +484,9 @@ baryon_project_kernel(
       const bool here = next < last; // else the next item's first tile (uniform)
       if constexpr (PF_J)
         if (here) buf[1].load(Pj, next, tile_sites(next, last), lane);
+      // vmcnt(9): everything issued before P_j's loads just issued, P_i's
+      // tile included, has completed
+      if constexpr (DMA_I) dma_wait_keep<Tile<C>::PER_LANE>();
       if (PREFETCH && (here || more)) {
         const size_t st = here ? next : first_next;
         const int n = here ? tile_sites(next, last) : tile_sites(first_next, last_next);

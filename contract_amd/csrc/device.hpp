@@ -306,8 +306,9 @@ __device__ __forceinline__ void dma_tile_pair_split(C *__restrict__ lds_i, C *__
 // LDS instructions execute in issue order, so finishing the earlier ones is
 // enough).  __syncthreads() is a workgroup release fence, which also waits for
 // every HBM -> LDS load in flight (vmcnt(0)); a kernel that keeps dma_tile
-// loads in flight across such a point uses this instead, and the compiler
-// still waits for a dma_tile before the first read of its destination.
+// loads in flight across such a point uses this instead, and waits for a
+// dma_tile itself (dma_wait, dma_wait_keep) before the first read of its
+// destination.
 __device__ __forceinline__ void wave_sync()
 {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -317,15 +318,31 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // Waits for every HBM -> LDS load (dma_tile) of the wave, and orders the
-// LDS reads that follow after it.  The compiler normally inserts this wait
-// before the first read of a dma_tile destination, but not in every loop
-// shape (the meson grid's wave-local kernel read its tiles with no wait), so
-// a kernel whose only loads in flight at that point are its tile loads
-// states it.  Out-of-range lanes of a dma_tile write zeros (measured).
+// LDS reads that follow after it.  The compiler sometimes inserts this wait
+// before the first read of a dma_tile destination, but not reliably (the
+// meson grid's wave-local kernel read its tiles with no wait, and so did the
+// staged baryon kernels once the module changed), so every reader states
+// it.  Out-of-range lanes of a dma_tile write zeros (measured).
 __device__ __forceinline__ void dma_wait()
 {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// dma_wait that leaves the KEEP most recently issued vector-memory
+// operations in flight (vmcnt(KEEP); loads count down in issue order): for a
+// dma_tile followed by at most KEEP loads or stores the wave does not need
+// yet.  Every kernel that reads a dma_tile destination states its wait: the
+// compiler's own placement of one is not reliable (an unrelated kernel added
+// to the same module once turned a generic-to-LDS cast into a null-checked
+// one, and the wait before the first read of the destination was dropped).
+template <int KEEP> __device__ __forceinline__ void dma_wait_keep()
+{
+  static_assert(KEEP >= 0 && KEEP < 64, "vmcnt has six bits");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0x0f70 | (KEEP & 15) | ((KEEP >> 4) << 14)); // vmcnt(KEEP)
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }

@@ -127,6 +127,41 @@ def test_larger_volume_and_empty(dev, medium):
     assert (out.cpu().numpy() == 3.0).all()
 
 
+@pytest.fixture(scope="module")
+def multi_tile():
+    """16^4 sites: 16384 four-site tiles, several per wave on any MI355X
+    launch (at most 8 waves per CU), so the kernels' steady-state pipelines
+    (next-tile loads in flight across steps) are exercised, not only the
+    first tile of a run."""
+    dims = (16, 16, 16, 16)
+    return [oracle.synthetic_propagator(dims, s, 0.25, t0=5) for s in (31, 32, 33)]
+
+
+def test_multi_tile_runs(dev, multi_tile):
+    torch, pc, cuda = dev
+    pi, pj, pn = multi_tile
+    d = [up(torch, cuda, p) for p in multi_tile]
+    for t in range(6):
+        got = pc.baryonTwoPoint(*d, t, 5, 5, 8).cpu().numpy()
+        assert rel(got, oracle.baryon_two_point(pi, pj, pn, t, 5, 5, 8)) < TOL, t
+    got = pc.baryonTwoPoint(*d, 2, 13, 10, 3).cpu().numpy()  # runtime Gij form
+    assert rel(got, oracle.baryon_two_point(pi, pj, pn, 2, 13, 10, 3)) < TOL
+    half = 0.5 * (np.eye(4) - oracle.gamma(8))
+    got = pc.baryonTwoPoint(*d, 0, 5, 5, half).cpu().numpy()
+    assert rel(got, oracle.baryon_general_two_point(pi, pj, pn, 0, 5, 5, half)) < TOL
+    got = pc.mesonTwoPoint(d[0], d[1], 5, 8).cpu().numpy()
+    assert rel(got, oracle.meson_two_point(pi, pj, 5, 8)) < TOL
+    src = pc.mesonAllSourceTwoPoint(d[0], d[1], 5).cpu().numpy()
+    snk = pc.mesonAllSinkTwoPoint(d[0], d[1], 8).cpu().numpy()
+    assert rel(src, oracle.meson_all_source_two_point(pi, pj, 5)) < TOL
+    assert rel(snk, oracle.meson_all_sink_two_point(pi, pj, 8)) < TOL
+    got = pc.baryonDiquark(d[0], d[1], 5, 5).cpu().numpy()
+    assert rel(got, oracle.baryon_diquark(pi, pj, 5, 5)) < TOL
+    pc.baryonSequentialTwoPoint(*d, 0, 2, 5, 5, 8)
+    want = oracle.baryon_sequential_two_point(pi, pj, pn, 0, 2, 5, 5, 8)
+    assert rel(d[2].cpu().numpy(), want[2]) < TOL
+
+
 def test_extreme_magnitudes(dev):
     """Steep exp(-0.8|t|) decay: ~20 decades across the lattice, still FP64-exact."""
     torch, pc, cuda = dev
