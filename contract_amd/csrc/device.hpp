@@ -108,26 +108,62 @@ template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s)
 __device__ __forceinline__ double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }
 __device__ __forceinline__ float shfl_xor_d(float v, int mask) { return __shfl_xor(v, mask, 64); }
 
-template <typename C> __device__ __forceinline__ C shfl_xor_c(C v, int mask)
+// Lane exchanges inside a 16-lane row on the vector ALUs (DPP moves) rather
+// than through the LDS crossbar (ds_bpermute, what __shfl_xor compiles to):
+// the reductions below are chains of dependent exchanges, and a DPP move
+// costs a VALU issue where a permute costs an LDS round trip.  gfx9 DPP
+// controls: quad_perm [1,0,3,2] = 0xb1 is xor 1, [2,3,0,1] = 0x4e is xor 2;
+// row_ror:n = 0x120 + n, where lane i reads lane (i - n) mod 16 of its row,
+// so row_ror:8 is xor 8, and xor 4 is row_ror:12 on the lanes with bit 2
+// clear and row_ror:4 on the others.
+template <int CTRL> __device__ __forceinline__ int dpp_i32(int v)
 {
-  return cmake<C>(shfl_xor_d(v.x, mask), shfl_xor_d(v.y, mask));
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_r(float v)
+{
+  return __builtin_bit_cast(float, dpp_i32<CTRL>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_r(double v)
+{
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)dpp_i32<CTRL>((int)(unsigned)b);
+  const unsigned hi = (unsigned)dpp_i32<CTRL>((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// the value of lane q ^ M of this lane's 16-lane row (q = lane & 15)
+template <int M, typename R> __device__ __forceinline__ R lane_xor(R v, int q)
+{
+  static_assert(M == 1 || M == 2 || M == 4 || M == 8, "an exchange inside a 16-lane row");
+  if constexpr (M == 1) return dpp_r<0xb1>(v);
+  else if constexpr (M == 2) return dpp_r<0x4e>(v);
+  else if constexpr (M == 8) return dpp_r<0x128>(v);
+  else {
+    const R up = dpp_r<0x12c>(v), down = dpp_r<0x124>(v);
+    return (q & 4) ? down : up;
+  }
 }
 
-// Sum over the 16 lanes of a site group; every lane of the group gets the sum.
+template <int M, typename C> __device__ __forceinline__ C shfl_xor_c(C v)
+{
+  static_assert(M != 4, "xor 4 depends on the lane: shfl_xor4_c");
+  return cmake<C>(lane_xor<M>(v.x, 0), lane_xor<M>(v.y, 0));
+}
+template <typename C> __device__ __forceinline__ C shfl_xor4_c(C v, int q)
+{
+  return cmake<C>(lane_xor<4>(v.x, q), lane_xor<4>(v.y, q));
+}
+
+// Sum over the 16 lanes of a site group; every lane of the group gets the
+// same bits.  The xor-4 step is a rotation by 4: after the xor-8 step lanes
+// i and i + 8 hold equal values, so every lane adds the same two values as
+// the xor butterfly would.
 template <typename C> __device__ __forceinline__ C group16_allreduce(C v)
 {
-#pragma unroll
-  for (int m = 8; m >= 1; m >>= 1) v = cadd(v, shfl_xor_c(v, m));
-  return v;
-}
-
-// Sum over the lanes that differ only in the bits of `span` (e.g. 12 = bits 2,3).
-template <typename C, int SPAN> __device__ __forceinline__ C group_allreduce_bits(C v)
-{
-#pragma unroll
-  for (int m = 8; m >= 1; m >>= 1)
-    if (SPAN & m) v = cadd(v, shfl_xor_c(v, m));
-  return v;
+  v = cadd(v, shfl_xor_c<8>(v));
+  v = cadd(v, cmake<C>(dpp_r<0x124>(v.x), dpp_r<0x124>(v.y)));
+  v = cadd(v, shfl_xor_c<2>(v));
+  return cadd(v, shfl_xor_c<1>(v));
 }
 
 // Reduce-scatter over a 16-lane group: every lane holds v[0..15]; on return
@@ -142,7 +178,7 @@ template <typename C> __device__ __forceinline__ C group16_reduce_scatter(C (&v)
     for (int k = 0; k < 8; ++k) {
       C keep = hi ? v[k + 8] : v[k];
       C send = hi ? v[k] : v[k + 8];
-      a8[k] = cadd(keep, shfl_xor_c(send, 8));
+      a8[k] = cadd(keep, shfl_xor_c<8>(send));
     }
   }
   C a4[4];
@@ -152,7 +188,7 @@ template <typename C> __device__ __forceinline__ C group16_reduce_scatter(C (&v)
     for (int k = 0; k < 4; ++k) {
       C keep = hi ? a8[k + 4] : a8[k];
       C send = hi ? a8[k] : a8[k + 4];
-      a4[k] = cadd(keep, shfl_xor_c(send, 4));
+      a4[k] = cadd(keep, shfl_xor4_c(send, q));
     }
   }
   C a2[2];
@@ -162,13 +198,13 @@ template <typename C> __device__ __forceinline__ C group16_reduce_scatter(C (&v)
     for (int k = 0; k < 2; ++k) {
       C keep = hi ? a4[k + 2] : a4[k];
       C send = hi ? a4[k] : a4[k + 2];
-      a2[k] = cadd(keep, shfl_xor_c(send, 2));
+      a2[k] = cadd(keep, shfl_xor_c<2>(send));
     }
   }
   const bool hi = q & 1;
   C keep = hi ? a2[1] : a2[0];
   C send = hi ? a2[0] : a2[1];
-  return cadd(keep, shfl_xor_c(send, 1));
+  return cadd(keep, shfl_xor_c<1>(send));
 }
 
 // ---------------------------------------------------------------------------
@@ -521,9 +557,9 @@ template <typename T> __device__ __forceinline__ T sel4(const T (&a)[4], int i) 
 template <typename C> __device__ __forceinline__ C reduce_scatter4_hi(const C (&r)[4], int q)
 {
   const bool h1 = q & 8, h0 = q & 4;
-  C a0 = cadd(h1 ? r[2] : r[0], shfl_xor_c(h1 ? r[0] : r[2], 8));
-  C a1 = cadd(h1 ? r[3] : r[1], shfl_xor_c(h1 ? r[1] : r[3], 8));
-  return cadd(h0 ? a1 : a0, shfl_xor_c(h0 ? a0 : a1, 4));
+  C a0 = cadd(h1 ? r[2] : r[0], shfl_xor_c<8>(h1 ? r[0] : r[2]));
+  C a1 = cadd(h1 ? r[3] : r[1], shfl_xor_c<8>(h1 ? r[1] : r[3]));
+  return cadd(h0 ? a1 : a0, shfl_xor4_c(h0 ? a0 : a1, q));
 }
 
 // Same over the lanes that differ in bits 0 and 1 (y = q & 3): lane y returns
@@ -531,9 +567,9 @@ template <typename C> __device__ __forceinline__ C reduce_scatter4_hi(const C (&
 template <typename C> __device__ __forceinline__ C reduce_scatter4_lo(const C (&r)[4], int q)
 {
   const bool h1 = q & 2, h0 = q & 1;
-  C a0 = cadd(h1 ? r[2] : r[0], shfl_xor_c(h1 ? r[0] : r[2], 2));
-  C a1 = cadd(h1 ? r[3] : r[1], shfl_xor_c(h1 ? r[1] : r[3], 2));
-  return cadd(h0 ? a1 : a0, shfl_xor_c(h0 ? a0 : a1, 1));
+  C a0 = cadd(h1 ? r[2] : r[0], shfl_xor_c<2>(h1 ? r[0] : r[2]));
+  C a1 = cadd(h1 ? r[3] : r[1], shfl_xor_c<2>(h1 ? r[1] : r[3]));
+  return cadd(h0 ? a1 : a0, shfl_xor_c<1>(h0 ? a0 : a1));
 }
 
 __device__ __forceinline__ int sparse_col(const SparseSpin &g, int row) { return sel4(g.col, row); }
