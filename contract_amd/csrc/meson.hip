@@ -45,9 +45,18 @@ void launch_meson(const MesonArgs &a, hipStream_t stream)
 
 // meson_all_source (Gij fixed, Gkl = G(g)) and meson_all_sink (Gkl fixed,
 // Gij = G(g)) for all g at once (include/meson_all.cuh:66-72,96-102).  Lane
-// (i, l) forms the four colour traces that any of the sixteen running gammas
-// can select, weights them, and a 16-lane reduce-scatter leaves correlator g
-// in lane g.  SINK is compile-time: one form's registers per kernel.
+// (i, l) forms the four colour traces any of the sixteen running gammas can
+// select,
+//   source: K_il[k] = tr(P_i[i][k], P_j[j(i)][l])   (j(i) the fixed column)
+//   sink:   K_il[j] = tr(P_i[i][k(l)], P_j[j][l])   (k(l) the fixed column),
+// posts them to LDS (the site's consumed P_i area), and lane g then gathers
+// the sixteen it needs for correlator g:
+//   source: c_g = sum_l G_g[l] sum_i Gij[i] K_il[col_g(l)]
+//   sink:   c_g = sum_i G_g[i] sum_l Gkl[l] K_il[col_g(i)]
+// (G_g[r] and col_g(r) the value and column of row r of (g5 G(g))^T).  One
+// LDS hand-over instead of a 16-lane reduce-scatter of sixteen weighted,
+// selected terms per lane.  SINK is compile-time: one form's registers per
+// kernel.
 template <bool SINK> __global__ __launch_bounds__(WAVE) void meson_all_kernel(MesonAllArgs a)
 {
   using C = double2;
@@ -55,15 +64,27 @@ template <bool SINK> __global__ __launch_bounds__(WAVE) void meson_all_kernel(Me
   __shared__ C lds[2 * SITES_PER_WAVE * SL];
   const int lane = threadIdx.x;
   const int g = lane >> 4, q = lane & 15;
+  const int i = q >> 2, l = q & 3;
+  // this lane's gather (correlator q): the running gamma's value and column
+  // per summed spin row, and the fixed gamma's values (uniform)
+  C run_val[4], fix_val[4];
+  int run_col[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    run_val[r] = sparse_val<C>(a.run[q], r);
+    run_col[r] = sparse_col(a.run[q], r);
+    fix_val[r] = make_double2(a.fixed.re[r], a.fixed.im[r]);
+  }
+  // read once: a lane-indexed kernel argument is a memory load, and one in
+  // the step would make the output store wait for the next tile's loads
+  C *const out = static_cast<C *>(a.correl[q]);
   const C *P[2] = {(const C *)a.propag_i, (const C *)a.propag_j};
   persistent_tiles<2>(lds, P, a.volume, lane, [&](size_t first, int nsites) {
-    const C *si = lds + g * SL, *sj = lds + (SITES_PER_WAVE + g) * SL;
-    const int i = q >> 2, l = q & 3;
-    C x[9], y[9], K[4], v[16];
+    C *si = lds + g * SL;
+    const C *sj = lds + (SITES_PER_WAVE + g) * SL;
+    C x[9], y[9], K[4];
     if constexpr (!SINK) {
-      // fixed row pair (i, j = a(i)); running k
-      const int j = sparse_col(a.fixed, i);
-      load_block(y, sj, j, l);
+      load_block(y, sj, sparse_col(a.fixed, i), l);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         load_block(x, si, i, k);
@@ -72,16 +93,8 @@ template <bool SINK> __global__ __launch_bounds__(WAVE) void meson_all_kernel(Me
         for (int e = 0; e < 9; ++e) t = cfma_conj(t, x[e], y[e]);
         K[k] = t;
       }
-      const C fa = sparse_val<C>(a.fixed, i);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int k = sparse_col(a.run[r], l);
-        const C sel = mux4(K[0], K[1], K[2], K[3], k);
-        v[r] = cmul(cmul(fa, sparse_val<C>(a.run[r], l)), sel);
-      }
     } else {
-      const int k = sparse_col(a.fixed, l);
-      load_block(x, si, i, k);
+      load_block(x, si, i, sparse_col(a.fixed, l));
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         load_block(y, sj, j, l);
@@ -90,16 +103,23 @@ template <bool SINK> __global__ __launch_bounds__(WAVE) void meson_all_kernel(Me
         for (int e = 0; e < 9; ++e) t = cfma_conj(t, x[e], y[e]);
         K[j] = t;
       }
-      const C fb = sparse_val<C>(a.fixed, l);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int j = sparse_col(a.run[r], i);
-        const C sel = mux4(K[0], K[1], K[2], K[3], j);
-        v[r] = cmul(cmul(sparse_val<C>(a.run[r], i), fb), sel);
-      }
     }
-    const C c = group16_reduce_scatter(v, q);
-    if (g < nsites) static_cast<C *>(a.correl[q])[first + g] = c;
+    wave_sync(); // every lane's tile reads are done: the site's P_i area takes K
+#pragma unroll
+    for (int k = 0; k < 4; ++k) si[q * 4 + k] = K[k]; // K_il[k] at slot 4 (4i + l) + k
+    wave_sync();
+    C c = czero<C>();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { // r = l (source) or i (sink): the row the running gamma weights
+      C t = czero<C>();
+#pragma unroll
+      for (int m = 0; m < 4; ++m) { // m = i (source) or l (sink): the row the fixed gamma weights
+        const int il = SINK ? 4 * r + m : 4 * m + r;
+        t = cfma(t, fix_val[m], si[il * 4 + run_col[r]]);
+      }
+      c = cfma(c, run_val[r], t);
+    }
+    if (g < nsites) out[first + g] = c;
   });
 }
 
