@@ -417,15 +417,26 @@ baryon_project_kernel(
     const size_t left = end - s;
     return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
   };
-  // FP32 site-scaled storage: e_i + e_j + e_n of this lane's site in the
-  // tile at st (padded lanes repeat the last site; their R is zero)
+  // FP32 site-scaled storage: the site exponents e_i, e_j, e_n of this
+  // lane's site in the tile at st (padded lanes repeat the last site; their
+  // R is zero), kept apart and summed where the step applies them: a sum
+  // formed as they load would wait for them right after the next tile's
+  // loads are issued, and so (loads count down in issue order) for the
+  // whole prefetch
+  struct SiteExp {
+    int i = 0, j = 0, n = 0;
+  };
   auto site_exp = [&](const ProjectReq &rq, size_t st, int n) {
     const size_t k = st + (size_t)(g < n ? g : n - 1);
-    if constexpr (SHARE) return 2 * (int)rq.exp_i[k] + (int)rq.exp_j[k];
-    return (int)rq.exp_i[k] + (int)rq.exp_j[k] + (int)rq.exp_n[k];
+    SiteExp e;
+    e.i = rq.exp_i[k];
+    e.j = rq.exp_j[k];
+    if constexpr (!SHARE) e.n = rq.exp_n[k];
+    return e;
   };
+  auto exp_sum = [](const SiteExp &e) { return SHARE ? 2 * e.i + e.j : e.i + e.j + e.n; };
   Tile<C> buf[NP];
-  int ex_buf = 0; // exponent sum of the tile in buf
+  SiteExp ex_buf; // site exponents of the tile in buf
   if constexpr (PREFETCH) {
     const ProjectReq &pa = args.req[r];
     const int n = tile_sites(first, last);
@@ -479,7 +490,8 @@ baryon_project_kernel(
         for (int p = 0; p < NL; ++p) buf[p].store(lds[p], lane);
       }
       step_sync();
-      int ex = ex_buf;
+      int ex = 0;
+      const SiteExp ex_tile = ex_buf;
       const size_t next = s + SITES_PER_WAVE;
       const bool here = next < last; // else the next item's first tile (uniform)
       if constexpr (PF_J)
@@ -546,6 +558,7 @@ baryon_project_kernel(
           R = cfma(R, make_double2(a.w_re[t], a.w_im[t]), make_double2((double)rt.x, (double)rt.y));
         }
       }
+      if constexpr (SCALED) ex = exp_sum(ex_tile);
       if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
       if constexpr (DMA_I) { // a ragged tile's padded sites read stale P_i slots
         R = g < nhere ? R : make_double2(0.0, 0.0);

@@ -58,11 +58,14 @@ __global__ __launch_bounds__(WAVE) void threepoint_project_kernel(ProjectArgs ar
     const size_t left = end - s;
     return left < (size_t)SITES_PER_WAVE ? (int)left : SITES_PER_WAVE;
   };
-  // site-scaled storage: e_F + e_Sigma of this lane's site (padded lanes
-  // repeat the last site; their M is zero)
+  // site-scaled storage: e_F and e_Sigma of this lane's site (padded lanes
+  // repeat the last site; their M is zero), summed where the step applies
+  // them: a sum formed as they load would wait for them right after the next
+  // tile's loads are issued, and so (loads count down in issue order) for
+  // the whole prefetch
   auto site_exp = [&](size_t st, int n) {
     const size_t k = st + (size_t)(g < n ? g : n - 1);
-    return (int)pa.exp_i[k] + (int)pa.exp_j[k];
+    return make_int2(pa.exp_i[k], pa.exp_j[k]);
   };
 
   PhaseWalk<NT> walk;
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(WAVE) void threepoint_project_kernel(ProjectArgs ar
     for (int u = 0; u < 4; ++u) acc[tt][u] = dbl4{0.0, 0.0, 0.0, 0.0};
 
   Tile<C> bf, bs;
-  int ex_buf = 0;
+  int2 ex_buf = make_int2(0, 0);
   {
     const int n = tile_sites(first, last);
     bf.load(F, first, n, lane);
@@ -85,7 +88,8 @@ __global__ __launch_bounds__(WAVE) void threepoint_project_kernel(ProjectArgs ar
     bf.store(lds[0], lane);
     bs.store(lds[1], lane);
     __syncthreads();
-    int ex = ex_buf;
+    int ex = 0;
+    const int2 ex_tile = ex_buf;
     const size_t next = s + SITES_PER_WAVE;
     if (next < last) {
       const int n = tile_sites(next, last);
@@ -108,6 +112,7 @@ __global__ __launch_bounds__(WAVE) void threepoint_project_kernel(ProjectArgs ar
       for (int e = 0; e < 9; ++e) m = cfma_conj(m, fb[e], sa[e]);
     }
     double2 R = make_double2((double)m.x, (double)m.y);
+    if constexpr (SCALED) ex = ex_tile.x + ex_tile.y;
     if constexpr (sizeof(Real) == 4) R = make_double2(ldexp(R.x, ex), ldexp(R.y, ex));
     project_accumulate<NT>(acc, args.np, pa, tab, R, walk); // padded lanes carry M = 0
     walk.advance(pa);
